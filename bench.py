@@ -1,0 +1,146 @@
+#!/usr/bin/env python3
+"""Throughput of the EK-PNP coupled step on one MI355X.
+
+A step is one coupled time step (fused LBM collide-stream of fluid + both ion
+species, forward x-FFT, wall-normal tridiagonal solve, inverse x-FFT) over the
+whole workload, with every input resident in HBM.  Default workload: the
+single 4096 x 1024 channel (tau 0.6, zeta -2, kappa h 32, E_x 5e-5, FP32
+deviation populations, FP64 Poisson and reductions), seeded cold start.
+
+Prints one JSON line (see the keys below).  value = MLUPS, lattice updates
+per second over the whole job / 1e6 (main.cu:248-250 counts all NX*NY nodes
+per step; so does this).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ek-pnp_amd"))
+
+METRIC = "MLUPS (% HBM peak); time-to-steady @res 1e-8, surrogate-warm vs cold"
+HBM_PEAK_GBS = 8000.0
+
+# algorithmic bytes per lattice update
+#   populations 27 x (read + write); coupling in FP64: psi read + rho_e write
+#   by the LBM kernel, rho_e read + psi write by the Poisson solve
+def alg_bytes(precision):
+    pop = 4 if precision == "fp32" else 8
+    lbm = 27 * 2 * pop + 16
+    return lbm, lbm + 16
+
+
+def workload(name):
+    import ekpnp
+
+    if name == "c1":
+        p = ekpnp.lattice_params(4096, 1024, tau=0.6, zeta=-2.0, kappa_h=32.0, e_x=5e-5)
+        return [p], "fp32", 1, "single channel 4096x1024, tau 0.6, zeta -2, kappa_h 32, E_x 5e-5"
+    if name == "c0":
+        p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
+        return [p], "fp64", 0, "single channel 64x128, tau 0.8, zeta -1, kappa_h 8, E_x 1e-4, FP64"
+    if name == "c4":
+        p = ekpnp.lattice_params(8192, 2048, tau=0.8, zeta=-1.5, kappa_h=64.0, e_x=1e-4)
+        return [p], "fp32", 4, "single channel 8192x2048, kappa_h 64, E_x 1e-4, uniform zeta -1.5"
+    raise SystemExit(f"unknown config {name}")
+
+
+def cpu_baseline(params, seconds_target=15.0):
+    """The FP64 oracle on the host cores, bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    p = params[0]
+    o = O.Oracle(p)
+    o.initialization(0, 1, 0)
+    o.init_equilibrium()
+    t1 = o.time_steps(1)
+    n = max(1, min(200, int(seconds_target / max(t1, 1e-6))))
+    t = o.time_steps(n)
+    nodes = p.nx * p.ny
+    return {
+        "value": round(n * nodes / t / 1e6, 3),
+        "unit": "MLUPS",
+        "cores": O.max_threads(),
+        "kind": "port",
+        "sample": f"FP64 oracle, {p.nx}x{p.ny} channel, {n} coupled steps after 1 warm-up step "
+                  f"({t:.1f} s), OpenMP over rows",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--config", default="c1")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--stage-steps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import ekpnp
+
+    params, precision, seed, desc = workload(args.config)
+    nodes = sum(p.nx * p.ny for p in params)
+    print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
+          file=sys.stderr, flush=True)
+    s = ekpnp.Solver(params, precision=precision)
+    s.initialization(ekpnp.COLD_NOISE, seed)
+    s.init_equilibrium()
+    s.step(args.warmup)
+    s.synchronize()
+    t0 = time.perf_counter()
+    s.step(args.steps)
+    s.synchronize()
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    mlups = args.steps * nodes / dt / 1e6
+    print(f"bench: timed {args.steps} steps in {dt:.3f} s", file=sys.stderr, flush=True)
+
+    # per-kernel durations, HIP events on the library's stream
+    st = s.time_stages(args.stage_steps)
+    red = s.reductions()
+    s.close()
+    lbm_b, coupled_b = alg_bytes(precision)
+    achieved = lbm_b * nodes / (st["lbm"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 populations, f64 Poisson" if precision == "fp32" else "f64",
+        "data": "synthetic (seeded cold start: c0 = 1 with U(-1e-3, 1e-3) noise)",
+        "config": {"workload": desc, "nodes": nodes, "precision": precision},
+        "hbm_frac_coupled": round(mlups * 1e6 * coupled_b / (HBM_PEAK_GBS * 1e9), 4),
+        "stages_ms": {k: round(v, 5) for k, v in st.items()},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_lbm (fused collide-stream)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+        },
+        "finite": bool(red[:, 6].sum() == 0),
+    }
+    if not args.no_cpu_baseline:
+        print("bench: cpu baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(params)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,947 @@
+// ekpnp_api.cpp -- host side of the C ABI declared in include/ekpnp.h.
+//
+// The context owns every device allocation for all channels; per-channel
+// sub-arrays are addressed through ChanDesc offsets.  Nothing here computes
+// on the host except parameter bookkeeping, input generation (the seeded cold
+// start) and the D2H formatting of outputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/ekpnp.h"
+#include "../../include/ekpnp_seed.h"
+#include "ekpnp_internal.h"
+
+using namespace ekpnp;
+
+namespace {
+
+const double W[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
+                     1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+
+thread_local std::string g_create_error;
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+int pow2_ceil(int v) {
+  int p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+struct ChannelTiles {
+  int fft_first, fft_count;  // range in the FFT tile table
+  int tri_first, tri_count;  // range in the tridiagonal tile table
+};
+
+}  // namespace
+
+struct ekpnp_ctx {
+  std::vector<ekpnp_channel_params> params;
+  ekpnp_options opt{};
+  int nchan = 0;
+  int elem = 4;  // bytes per population value
+  std::vector<ChanDesc> desc;
+  std::vector<ChannelTiles> ctiles;
+  long long total_nodes = 0, total_spec = 0, total_pops = 0;
+  int total_nx = 0;
+  // host copies
+  std::vector<int2> lbm_tiles;
+  std::vector<int4> fft_tiles;
+  std::vector<int2> tri_tiles;
+  std::vector<int> first_tile;
+  std::vector<double> walls;  // 6 planes x total_nx
+  // device
+  void* lat[2] = {nullptr, nullptr};
+  ChanDesc* d_desc = nullptr;
+  int2* d_lbm_tiles = nullptr;
+  int4* d_fft_tiles = nullptr;
+  int2* d_tri_tiles = nullptr;
+  int* d_first_tile = nullptr;
+  double* d_rho_e = nullptr;
+  double* d_psi = nullptr;
+  double* d_macro = nullptr;
+  double* d_walls = nullptr;
+  double2* d_tw = nullptr;
+  double2* d_spec = nullptr;
+  double* d_ux_prev = nullptr;
+  double* d_partials = nullptr;
+  double* d_red = nullptr;
+  size_t dev_bytes = 0;
+  hipStream_t stream = nullptr;
+  StepBuffers sb{};
+  // state
+  int cur = 0;          // lattice holding the current state
+  bool state_q = false; // current lattice holds unshifted pre-collision populations
+  bool has_fields = false, has_pops = false;
+  long long steps = 0;
+  long long snapshot_step = -1;
+  std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+  std::string err;
+};
+
+namespace {
+
+int fail(ekpnp_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_OR_FAIL(ctx, call)                                                     \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return fail((ctx), EKPNP_ERR_HIP,                                            \
+                  std::string(#call) + ": " + hipGetErrorString(e_));              \
+  } while (0)
+
+std::string validate(const ekpnp_channel_params& p, int ch) {
+  char buf[256];
+  auto bad = [&](const char* what) {
+    snprintf(buf, sizeof buf, "channel %d: %s", ch, what);
+    return std::string(buf);
+  };
+  if (p.nx < 2) return bad("nx < 2");
+  if (p.ny < 4) return bad("ny < 4");
+  if (is_pow2(p.nx) && p.nx > MAX_NX) return bad("power-of-two nx above 8192");
+  if (!is_pow2(p.nx) && p.nx > MAX_NX_DFT) return bad("non-power-of-two nx above 2048");
+  if (p.ny - 2 > MAX_INTERIOR_ROWS) return bad("ny above 2050");
+  if (!(p.dx > 0 && p.dy > 0 && p.dt > 0 && p.cfl > 0 && p.cs2 > 0)) return bad("non-positive dx/dy/dt/cfl/cs2");
+  if (!(p.nu > 0 && p.diff_pos > 0 && p.diff_neg > 0)) return bad("non-positive viscosity or diffusivity");
+  if (!(p.magic_f > 0 && p.magic_pos > 0 && p.magic_neg > 0)) return bad("non-positive TRT magic parameter");
+  if (!(p.rho0 > 0)) return bad("non-positive rho0");
+  if (!(p.lx > 0)) return bad("non-positive lx");
+  if (!std::isfinite(p.poisson_coef) || !std::isfinite(p.force_coef)) return bad("non-finite coefficient");
+  return std::string();
+}
+
+ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
+  ChanDesc d{};
+  d.nx = p.nx;
+  d.ny = p.ny;
+  d.nk = p.nx / 2 + 1;
+  d.flags = (int)p.flags;
+  d.pow2 = (is_pow2(p.nx) && p.nx >= 4) ? 1 : 0;
+  const double cs2 = p.cs2, dt = p.dt;
+  // relaxation rates, LBM.cu:336-341
+  const double omega_plus = 1.0 / (p.nu / cs2 / dt + 1.0 / 2.0) / dt;
+  const double omega_minus = 1.0 / (p.magic_f / (p.nu / cs2 / dt) + 1.0 / 2.0) / dt;
+  const double omega_c_minus = 1.0 / (p.diff_pos / cs2 / dt + 1.0 / 2.0) / dt;
+  const double omega_c_plus = 1.0 / (p.magic_pos / (p.diff_pos / cs2 / dt) + 1.0 / 2.0) / dt;
+  const double omega_T_minus = 1.0 / (p.diff_neg / cs2 / dt + 1.0 / 2.0) / dt;
+  const double omega_T_plus = 1.0 / (p.magic_neg / (p.diff_neg / cs2 / dt) + 1.0 / 2.0) / dt;
+  d.inv_cfl = 1.0 / p.cfl;
+  d.inv_cs2 = 1.0 / cs2;
+  d.om_pf = omega_plus * dt;
+  d.om_mf = omega_minus * dt;
+  d.om_pp = omega_c_plus * dt;
+  d.om_mp = omega_c_minus * dt;
+  d.om_pn = omega_T_plus * dt;
+  d.om_mn = omega_T_minus * dt;
+  d.src_p = 1.0 - 0.5 * dt * omega_plus;  // LBM.cu:745-746
+  d.src_m = 1.0 - 0.5 * dt * omega_minus;
+  d.dt = dt;
+  d.half_dt = 0.5 * dt;
+  if (precision == EKPNP_POP_FP32) {
+    d.rho_ref = p.rho0;
+    d.cp_ref = p.c_bulk;
+    d.cn_ref = p.c_bulk;
+  }
+  d.kp = p.mob_pos;
+  d.kn = p.mob_neg;
+  d.fcoef = p.force_coef;
+  d.e_app = p.e_applied;
+  d.exf = p.exf;
+  d.drift_app = (p.flags & EKPNP_FLAG_ION_DRIFT_APPLIED) ? p.e_applied : 0.0;
+  d.inv2dx = 0.5 / p.dx;
+  d.inv2dy = 0.5 / p.dy;
+  d.multis = 2.0 * p.rho0 * p.uw / cs2 * (1.0 / 9.0) / p.cfl;  // LBM.cu:812-813
+  d.multid = 2.0 * p.rho0 * p.uw / cs2 * (1.0 / 36.0) / p.cfl;
+  d.pc_dy2 = p.poisson_coef * p.dy * p.dy;
+  d.kdy_scale = 2.0 * M_PI / p.lx * p.dy;
+  // tridiagonal partition: C chunks of L rows, G = TRI_WG / C modes per WG
+  const int M = p.ny - 2;
+  int C = pow2_ceil((M + 15) / 16);
+  C = std::max(8, std::min(64, C));
+  d.tri_C = C;
+  d.tri_L = (M + C - 1) / C;
+  d.tri_G = TRI_WG / C;
+  return d;
+}
+
+StepBuffers make_buffers(ekpnp_ctx* c) {
+  StepBuffers b{};
+  b.chans = c->d_desc;
+  b.lbm_tiles = c->d_lbm_tiles;
+  b.n_lbm_tiles = (int)c->lbm_tiles.size();
+  b.fft_tiles = c->d_fft_tiles;
+  b.n_fft_tiles = (int)c->fft_tiles.size();
+  b.tri_tiles = c->d_tri_tiles;
+  b.n_tri_tiles = (int)c->tri_tiles.size();
+  b.rho_e = c->d_rho_e;
+  b.psi = c->d_psi;
+  b.macro = c->d_macro;
+  b.macro_stride = c->total_nodes;
+  b.walls = c->d_walls;
+  b.walls_stride = c->total_nx;
+  b.tw = c->d_tw;
+  b.spec = c->d_spec;
+  b.ux_prev = c->d_ux_prev;
+  b.partials = c->d_partials;
+  b.red_out = c->d_red;
+  b.red_first_tile = c->d_first_tile;
+  b.n_channels = c->nchan;
+  return b;
+}
+
+template <typename T>
+hipError_t dalloc(ekpnp_ctx* c, T** p, size_t count) {
+  size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e == hipSuccess) c->dev_bytes += bytes;
+  return e;
+}
+
+void release(ekpnp_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+  void* ptrs[] = {c->lat[0], c->lat[1], c->d_desc, c->d_lbm_tiles, c->d_fft_tiles,
+                  c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
+                  c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  c->stream = nullptr;
+}
+
+// Poisson stages for all channels, or for one channel's tile ranges.
+hipError_t enqueue_poisson(ekpnp_ctx* c, int only_channel) {
+  StepBuffers b = c->sb;
+  if (only_channel >= 0) {
+    const ChannelTiles& t = c->ctiles[only_channel];
+    b.fft_tiles = c->d_fft_tiles + t.fft_first;
+    b.n_fft_tiles = t.fft_count;
+    b.n_fft_pow2 = c->desc[only_channel].pow2 ? t.fft_count : 0;
+    b.tri_tiles = c->d_tri_tiles + t.tri_first;
+    b.n_tri_tiles = t.tri_count;
+  }
+  hipError_t e = launch_poisson_fwd(b, c->stream);
+  if (e != hipSuccess) return e;
+  e = launch_poisson_tri(b, c->stream);
+  if (e != hipSuccess) return e;
+  return launch_poisson_inv(b, c->stream);
+}
+
+hipError_t enqueue_lbm(ekpnp_ctx* c, bool macro) {
+  hipError_t e = launch_lbm(c->opt.precision, c->opt.compute_fp64 != 0, c->state_q, macro, c->sb,
+                            c->lat[c->cur],
+                            c->lat[c->cur ^ 1], c->stream);
+  c->cur ^= 1;
+  c->state_q = false;
+  return e;
+}
+
+hipError_t enqueue_steps(ekpnp_ctx* c, int n, bool macro_last) {
+  for (int i = 0; i < n; ++i) {
+    hipError_t e = enqueue_lbm(c, macro_last && i == n - 1);
+    if (e != hipSuccess) return e;
+    e = enqueue_poisson(c, -1);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+// n steps from the P' state through a cached hipGraph
+int graph_steps(ekpnp_ctx* c, int n, bool macro_last) {
+  auto key = std::make_tuple(n, c->cur, macro_last ? 1 : 0, 0);
+  auto it = c->graphs.find(key);
+  if (it == c->graphs.end()) {
+    const int cur0 = c->cur;
+    HIP_OR_FAIL(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue_steps(c, n, macro_last);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+    c->cur = cur0;
+    if (e != hipSuccess) return fail(c, EKPNP_ERR_HIP, std::string("capture: ") + hipGetErrorString(e));
+    if (e2 != hipSuccess) return fail(c, EKPNP_ERR_HIP, std::string("end capture: ") + hipGetErrorString(e2));
+    hipGraphExec_t ex = nullptr;
+    hipError_t e3 = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e3 != hipSuccess) return fail(c, EKPNP_ERR_HIP, std::string("instantiate: ") + hipGetErrorString(e3));
+    it = c->graphs.emplace(key, ex).first;
+  }
+  HIP_OR_FAIL(c, hipGraphLaunch(it->second, c->stream));
+  c->cur ^= (n & 1);
+  return EKPNP_OK;
+}
+
+int check_channel(ekpnp_ctx* c, int ch) {
+  if (!c) return EKPNP_ERR_INVALID;
+  if (ch < 0 || ch >= c->nchan) return fail(c, EKPNP_ERR_INVALID, "channel index out of range");
+  return EKPNP_OK;
+}
+
+// psi wall rows of one channel from the wall planes
+hipError_t upload_psi_walls(ekpnp_ctx* c, int ch) {
+  const ChanDesc& d = c->desc[ch];
+  const double* lo = &c->walls[W_PSI_LO * (size_t)c->total_nx + d.wall_off];
+  const double* hi = &c->walls[W_PSI_HI * (size_t)c->total_nx + d.wall_off];
+  hipError_t e = hipMemcpyAsync(c->d_psi + d.fld_off, lo, d.nx * sizeof(double),
+                                hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return e;
+  e = hipMemcpyAsync(c->d_psi + d.fld_off + (long long)(d.ny - 1) * d.nx, hi, d.nx * sizeof(double),
+                     hipMemcpyHostToDevice, c->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c->stream);
+}
+
+}  // namespace
+
+extern "C" {
+
+void ekpnp_reference_params(ekpnp_channel_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->nx = 10;              // LBM.h:35
+  p->ny = 101;             // LBM.h:36
+  p->flags = EKPNP_FLAG_G0_FLUID_U;  // LBM.cu:493 as shipped
+  p->dx = 1.0e-6 / 100.0;  // LBM.h:42
+  p->dy = 1.0e-6 / 100.0;  // LBM.h:43
+  p->cfl = 0.01;           // LBM.h:50
+  p->dt = 0.01 * 1.0e-6 / 100.0;            // LBM.h:51
+  p->cs2 = 1.0 / 3.0 / (0.01 * 0.01);       // LBM.h:52
+  p->rho0 = 1000.0;        // LBM.h:53
+  p->nu = 0.889e-6;        // LBM.h:65
+  p->diff_pos = 1.0e-8;    // LBM.h:64
+  p->diff_neg = 1.0e-8;    // LBM.h:71
+  p->mob_pos = 4.245e-7;   // LBM.h:67
+  p->mob_neg = -4.245e-7;  // LBM.h:74
+  p->magic_f = 1.0e-6;     // LBM.h:110-112
+  p->magic_pos = 1.0e-6;
+  p->magic_neg = 1.0e-6;
+  p->psi_lo = -5.2574e-3;  // LBM.h:58
+  p->psi_hi = -5.2574e-3;  // LBM.h:60
+  p->cpos_lo = 1.2364549e-2;  // LBM.h:55-56
+  p->cpos_hi = 1.2364549e-2;
+  p->cneg_lo = 8.087639e-3;   // LBM.h:81-83
+  p->cneg_hi = 8.087639e-3;
+  p->c_bulk = 0.01;           // chargeinf, LBM.h:54
+  p->poisson_coef = 9.64e4 / 6.95e-10;  // convertCtoCharge/eps, LBM.h:63, 90
+  p->force_coef = 0.0;        // LBM.cu:388-389 as shipped
+  p->e_applied = 1.0e3;       // LBM.h:62
+  p->exf = 0.0;
+  p->uw = 0.0;
+  p->lx = 1.0e-7;             // LBM.h:40
+  p->init_noise = 0.0;
+}
+
+int ekpnp_lattice_params(ekpnp_channel_params* p, int nx, int ny, double tau, double zeta,
+                         double kappa_h, double e_x, double diff) {
+  if (!p || nx < 2 || ny < 4 || !(tau > 0.5) || !(diff > 0)) return EKPNP_ERR_INVALID;
+  std::memset(p, 0, sizeof(*p));
+  p->nx = nx;
+  p->ny = ny;
+  p->flags = 0;
+  p->dx = p->dy = p->dt = p->cfl = 1.0;
+  p->cs2 = 1.0 / 3.0;
+  p->rho0 = 1.0;
+  p->nu = (tau - 0.5) / 3.0;
+  p->diff_pos = p->diff_neg = diff;
+  p->mob_pos = diff;
+  p->mob_neg = -diff;
+  p->magic_f = p->magic_pos = p->magic_neg = 0.25;
+  p->psi_lo = p->psi_hi = zeta;
+  p->cpos_lo = p->cpos_hi = std::exp(-zeta);
+  p->cneg_lo = p->cneg_hi = std::exp(zeta);
+  p->c_bulk = 1.0;
+  const double h = 0.5 * (ny - 1);
+  const double kappa = kappa_h / h;
+  p->poisson_coef = 0.5 * kappa * kappa;
+  p->force_coef = p->poisson_coef;
+  p->e_applied = e_x;
+  p->exf = 0.0;
+  p->uw = 0.0;
+  p->lx = (double)nx;
+  p->init_noise = 1.0e-3;
+  return EKPNP_OK;
+}
+
+void ekpnp_default_options(ekpnp_options* o) {
+  if (!o) return;
+  o->precision = EKPNP_POP_FP32;
+  o->device = 0;
+  o->use_graph = 1;
+  o->compute_fp64 = 0;
+}
+
+const char* ekpnp_create_error(void) { return g_create_error.c_str(); }
+
+const char* ekpnp_last_error(const ekpnp_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekpnp_options* opt,
+                 ekpnp_ctx** out) {
+  g_create_error.clear();
+  if (!out) return EKPNP_ERR_INVALID;
+  *out = nullptr;
+  if (!channels || n_channels <= 0) {
+    g_create_error = "no channels";
+    return EKPNP_ERR_INVALID;
+  }
+  ekpnp_options o;
+  ekpnp_default_options(&o);
+  if (opt) o = *opt;
+  if (o.precision != EKPNP_POP_FP32 && o.precision != EKPNP_POP_FP64) {
+    g_create_error = "unknown precision";
+    return EKPNP_ERR_INVALID;
+  }
+  for (int i = 0; i < n_channels; ++i) {
+    std::string m = validate(channels[i], i);
+    if (!m.empty()) {
+      g_create_error = m;
+      return (m.find("above") != std::string::npos) ? EKPNP_ERR_UNSUPPORTED : EKPNP_ERR_INVALID;
+    }
+  }
+  hipError_t he = hipSetDevice(o.device);
+  if (he != hipSuccess) {
+    g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(he);
+    return EKPNP_ERR_HIP;
+  }
+  ekpnp_ctx* c = new ekpnp_ctx();
+  c->opt = o;
+  c->nchan = n_channels;
+  c->elem = (o.precision == EKPNP_POP_FP32) ? 4 : 8;
+  c->params.assign(channels, channels + n_channels);
+  c->desc.resize(n_channels);
+  c->ctiles.resize(n_channels);
+
+  // layout
+  long long pop = 0, fld = 0, spec = 0;
+  int wall = 0;
+  // every per-channel block starts on a 64-element boundary
+  auto pad64 = [](long long v) { return (v + 63) / 64 * 64; };
+  for (int i = 0; i < n_channels; ++i) {
+    ChanDesc d = make_desc(channels[i], o.precision);
+    const long long nn = (long long)d.nx * d.ny;
+    d.plane = pad64(nn);
+    d.pop_off = pop;
+    d.fld_off = fld;
+    d.spec_off = spec;
+    d.wall_off = wall;
+    d.tw_off = wall;
+    pop += 27 * d.plane;
+    fld += pad64(nn);
+    spec += pad64((long long)d.ny * d.nk);
+    wall += (int)pad64(d.nx);
+    c->desc[i] = d;
+  }
+  c->total_pops = pop;
+  c->total_nodes = fld;
+  c->total_spec = spec;
+  c->total_nx = wall;
+
+  // LBM tiles, channel-major
+  c->first_tile.resize(n_channels + 1);
+  for (int i = 0; i < n_channels; ++i) {
+    c->first_tile[i] = (int)c->lbm_tiles.size();
+    const long long nn = (long long)c->desc[i].nx * c->desc[i].ny;
+    for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
+  }
+  c->first_tile[n_channels] = (int)c->lbm_tiles.size();
+  // FFT tiles: power-of-two channels first
+  unsigned lds_pow2 = 16, lds_dft = 16;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i < n_channels; ++i) {
+      ChanDesc& d = c->desc[i];
+      if ((pass == 0) != (d.pow2 == 1)) continue;
+      const int M = d.ny - 2;
+      int rows = 1;
+      if (d.pow2) {
+        const int N = d.nx / 2;
+        rows = std::max(1, std::min(M, FFT_MAX_ELEMS / N));
+        lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(rows * N * sizeof(double2)));
+      } else {
+        lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
+      }
+      d.fft_rows = rows;
+      c->ctiles[i].fft_first = (int)c->fft_tiles.size();
+      for (int y0 = 1; y0 <= M; y0 += rows)
+        c->fft_tiles.push_back(make_int4(i, y0, std::min(rows, M + 1 - y0), 0));
+      c->ctiles[i].fft_count = (int)c->fft_tiles.size() - c->ctiles[i].fft_first;
+    }
+    if (pass == 0) c->sb.n_fft_pow2 = (int)c->fft_tiles.size();
+  }
+  const int n_fft_pow2 = c->sb.n_fft_pow2;
+  int tri_lmax = 16;
+  for (int i = 0; i < n_channels; ++i) {
+    const ChanDesc& d = c->desc[i];
+    c->ctiles[i].tri_first = (int)c->tri_tiles.size();
+    for (int k0 = 0; k0 < d.nk; k0 += d.tri_G) c->tri_tiles.push_back(make_int2(i, k0));
+    c->ctiles[i].tri_count = (int)c->tri_tiles.size() - c->ctiles[i].tri_first;
+    if (d.tri_L > 16) tri_lmax = 32;
+  }
+
+  // wall planes and twiddles
+  c->walls.assign(6 * (size_t)wall, 0.0);
+  std::vector<double2> tw((size_t)wall);
+  for (int i = 0; i < n_channels; ++i) {
+    const ekpnp_channel_params& p = channels[i];
+    const ChanDesc& d = c->desc[i];
+    for (int x = 0; x < d.nx; ++x) {
+      c->walls[W_PSI_LO * (size_t)wall + d.wall_off + x] = p.psi_lo;
+      c->walls[W_PSI_HI * (size_t)wall + d.wall_off + x] = p.psi_hi;
+      c->walls[W_CP_LO * (size_t)wall + d.wall_off + x] = p.cpos_lo;
+      c->walls[W_CP_HI * (size_t)wall + d.wall_off + x] = p.cpos_hi;
+      c->walls[W_CN_LO * (size_t)wall + d.wall_off + x] = p.cneg_lo;
+      c->walls[W_CN_HI * (size_t)wall + d.wall_off + x] = p.cneg_hi;
+      const double a = 2.0 * M_PI * (double)x / (double)d.nx;
+      tw[d.tw_off + x] = make_double2(std::cos(a), -std::sin(a));
+    }
+  }
+
+  // device allocations
+  auto alloc_all = [&]() -> hipError_t {
+    hipError_t e;
+    const size_t npop = (size_t)c->total_pops;
+    for (int k = 0; k < 2; ++k) {
+      e = hipMalloc(&c->lat[k], std::max<size_t>(npop * c->elem, 16));
+      if (e != hipSuccess) return e;
+      c->dev_bytes += npop * c->elem;
+    }
+    if ((e = dalloc(c, &c->d_desc, n_channels)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_lbm_tiles, c->lbm_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_fft_tiles, c->fft_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_tri_tiles, c->tri_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_first_tile, c->first_tile.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_rho_e, c->total_nodes)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_psi, c->total_nodes)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_macro, M_COUNT * (size_t)c->total_nodes)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_walls, c->walls.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_tw, tw.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_spec, c->total_spec)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_ux_prev, c->total_nodes)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_partials, 8 * c->lbm_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_red, 8 * (size_t)n_channels)) != hipSuccess) return e;
+    if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return e;
+    return hipSuccess;
+  };
+  he = alloc_all();
+  if (he != hipSuccess) {
+    g_create_error = std::string("allocation: ") + hipGetErrorString(he);
+    release(c);
+    delete c;
+    return (he == hipErrorOutOfMemory) ? EKPNP_ERR_NOMEM : EKPNP_ERR_HIP;
+  }
+  auto upload = [&]() -> hipError_t {
+    hipError_t e;
+    if ((e = hipMemcpy(c->d_desc, c->desc.data(), n_channels * sizeof(ChanDesc), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_lbm_tiles, c->lbm_tiles.data(), c->lbm_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_fft_tiles, c->fft_tiles.data(), c->fft_tiles.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_tri_tiles, c->tri_tiles.data(), c->tri_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_first_tile, c->first_tile.data(), c->first_tile.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_walls, c->walls.data(), c->walls.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    const size_t npop = (size_t)c->total_pops;
+    if ((e = hipMemset(c->lat[0], 0, npop * c->elem)) != hipSuccess) return e;
+    if ((e = hipMemset(c->lat[1], 0, npop * c->elem)) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_rho_e, 0, c->total_nodes * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_psi, 0, c->total_nodes * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_macro, 0, M_COUNT * c->total_nodes * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_ux_prev, 0, c->total_nodes * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_spec, 0, c->total_spec * sizeof(double2))) != hipSuccess) return e;
+    return hipDeviceSynchronize();
+  };
+  he = upload();
+  if (he != hipSuccess) {
+    g_create_error = std::string("upload: ") + hipGetErrorString(he);
+    release(c);
+    delete c;
+    return EKPNP_ERR_HIP;
+  }
+  c->sb = make_buffers(c);
+  c->sb.n_fft_pow2 = n_fft_pow2;
+  c->sb.fft_lds_pow2 = lds_pow2;
+  c->sb.fft_lds_dft = lds_dft;
+  c->sb.tri_lmax = tri_lmax;
+  for (int i = 0; i < n_channels; ++i) {
+    if (upload_psi_walls(c, i) != hipSuccess) {
+      g_create_error = "psi wall upload failed";
+      release(c);
+      delete c;
+      return EKPNP_ERR_HIP;
+    }
+  }
+  *out = c;
+  return EKPNP_OK;
+}
+
+void ekpnp_destroy(ekpnp_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->opt.device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  release(ctx);
+  delete ctx;
+}
+
+int ekpnp_set_wall_profile(ekpnp_ctx* ctx, int channel, const double* psi_lo, const double* psi_hi,
+                           const double* cpos_lo, const double* cpos_hi, const double* cneg_lo,
+                           const double* cneg_hi) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ChanDesc& d = ctx->desc[channel];
+  const double* src[6] = {psi_lo, psi_hi, cpos_lo, cpos_hi, cneg_lo, cneg_hi};
+  for (int a = 0; a < 6; ++a) {
+    if (!src[a]) continue;
+    double* dst = &ctx->walls[a * (size_t)ctx->total_nx + d.wall_off];
+    std::memcpy(dst, src[a], d.nx * sizeof(double));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_walls + a * (size_t)ctx->total_nx + d.wall_off, dst,
+                                    d.nx * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  }
+  // graphs read the wall planes through device pointers: nothing to rebuild
+  HIP_OR_FAIL(ctx, upload_psi_walls(ctx, channel));
+  return EKPNP_OK;
+}
+
+int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho, const double* ux,
+                     const double* uy, const double* cpos, const double* cneg, const double* psi) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (!rho || !ux || !uy || !cpos || !cneg) return fail(ctx, EKPNP_ERR_INVALID, "null field");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ChanDesc& d = ctx->desc[channel];
+  const size_t nn = (size_t)d.nx * d.ny;
+  const double* f[M_COUNT] = {rho, ux, uy, cpos, cneg};
+  for (int k = 0; k < M_COUNT; ++k)
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_macro + k * ctx->total_nodes + d.fld_off, f[k],
+                                    nn * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  std::vector<double> re(nn);
+  for (size_t i = 0; i < nn; ++i) re[i] = cpos[i] - cneg[i];
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_rho_e + d.fld_off, re.data(), nn * sizeof(double),
+                                  hipMemcpyHostToDevice, ctx->stream));
+  if (psi) {
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_psi + d.fld_off, psi, nn * sizeof(double),
+                                    hipMemcpyHostToDevice, ctx->stream));
+  } else {
+    HIP_OR_FAIL(ctx, enqueue_poisson(ctx, channel));
+  }
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->has_fields = true;
+  return EKPNP_OK;
+}
+
+int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (mode != EKPNP_INIT_COLD_NOISE && mode != EKPNP_INIT_REFERENCE)
+    return fail(ctx, EKPNP_ERR_INVALID, "unknown initialization mode");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  for (int ch = 0; ch < ctx->nchan; ++ch) {
+    const ekpnp_channel_params& p = ctx->params[ch];
+    const ChanDesc& d = ctx->desc[ch];
+    const size_t nn = (size_t)d.nx * d.ny;
+    std::vector<double> rho(nn, p.rho0), zero(nn, 0.0), cp(nn), cn(nn);
+    if (mode == EKPNP_INIT_REFERENCE) {
+      // gpu_initialization, LBM.cu:147-160
+      std::vector<double> psi(nn, p.psi_lo);
+      int rc = ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), zero.data(),
+                                zero.data(), psi.data());
+      if (rc) return rc;
+    } else {
+      const size_t T = ctx->total_nx;
+      ekpnp_cold_field(cp.data(), d.nx, d.ny, p.c_bulk, p.init_noise,
+                       &ctx->walls[W_CP_LO * T + d.wall_off], &ctx->walls[W_CP_HI * T + d.wall_off],
+                       seed, ch, 0);
+      ekpnp_cold_field(cn.data(), d.nx, d.ny, p.c_bulk, p.init_noise,
+                       &ctx->walls[W_CN_LO * T + d.wall_off], &ctx->walls[W_CN_HI * T + d.wall_off],
+                       seed, ch, 1);
+      int rc = ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), cp.data(), cn.data(),
+                                nullptr);
+      if (rc) return rc;
+    }
+  }
+  ctx->has_pops = false;
+  ctx->steps = 0;
+  ctx->snapshot_step = -1;
+  return EKPNP_OK;
+}
+
+int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "init_equilibrium before initialization/set_fields");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, launch_init_equilibrium(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  ctx->state_q = true;
+  ctx->has_pops = true;
+  return EKPNP_OK;
+}
+
+int ekpnp_stream_collide_save(ekpnp_ctx* ctx) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, enqueue_lbm(ctx, true));
+  ctx->steps++;
+  return EKPNP_OK;
+}
+
+int ekpnp_fast_poisson(ekpnp_ctx* ctx) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "no fields");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, enqueue_poisson(ctx, -1));
+  return EKPNP_OK;
+}
+
+int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (nsteps < 0) return fail(ctx, EKPNP_ERR_INVALID, "negative step count");
+  if (nsteps == 0) return EKPNP_OK;
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  int n = nsteps;
+  if (ctx->state_q) {
+    HIP_OR_FAIL(ctx, enqueue_steps(ctx, 1, n == 1));
+    n -= 1;
+  }
+  if (!ctx->opt.use_graph) {
+    HIP_OR_FAIL(ctx, enqueue_steps(ctx, n, true));
+  } else {
+    const int CHUNK = 64;
+    while (n > 0) {
+      const int k = n > CHUNK ? CHUNK : n;
+      const bool last = (k == n);
+      int rc = graph_steps(ctx, k, last);
+      if (rc) return rc;
+      n -= k;
+    }
+  }
+  ctx->steps += nsteps;
+  return EKPNP_OK;
+}
+
+int ekpnp_synchronize(ekpnp_ctx* ctx) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  return EKPNP_OK;
+}
+
+int ekpnp_get_field(ekpnp_ctx* ctx, int channel, int field, double* host_out) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (!host_out) return fail(ctx, EKPNP_ERR_INVALID, "null output");
+  if (field < 0 || field >= EKPNP_FIELD_COUNT) return fail(ctx, EKPNP_ERR_INVALID, "unknown field");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ChanDesc& d = ctx->desc[channel];
+  const ekpnp_channel_params& p = ctx->params[channel];
+  const size_t nn = (size_t)d.nx * d.ny;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  if (field <= EKPNP_FIELD_CNEG) {
+    HIP_OR_FAIL(ctx, hipMemcpy(host_out, ctx->d_macro + field * ctx->total_nodes + d.fld_off,
+                               nn * sizeof(double), hipMemcpyDeviceToHost));
+    return EKPNP_OK;
+  }
+  std::vector<double> psi(nn);
+  HIP_OR_FAIL(ctx, hipMemcpy(psi.data(), ctx->d_psi + d.fld_off, nn * sizeof(double), hipMemcpyDeviceToHost));
+  if (field == EKPNP_FIELD_PSI) {
+    std::memcpy(host_out, psi.data(), nn * sizeof(double));
+    return EKPNP_OK;
+  }
+  // E = -grad(psi), poisson.cu:145-174
+  const int nx = d.nx, ny = d.ny;
+  for (int y = 0; y < ny; ++y) {
+    for (int x = 0; x < nx; ++x) {
+      const int xm = (x + nx - 1) % nx, xp = (x + 1) % nx;
+      if (field == EKPNP_FIELD_EX) {
+        host_out[(size_t)y * nx + x] = (psi[(size_t)y * nx + xm] - psi[(size_t)y * nx + xp]) * (0.5 / p.dx);
+      } else {
+        int ya = y - 1, yb = y + 1;
+        if (y == 0) { ya = 0; yb = 2; }
+        if (y == ny - 1) { ya = ny - 3; yb = ny - 1; }
+        host_out[(size_t)y * nx + x] = (psi[(size_t)ya * nx + x] - psi[(size_t)yb * nx + x]) * (0.5 / p.dy);
+      }
+    }
+  }
+  return EKPNP_OK;
+}
+
+int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host_out) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (species < 0 || species > 2 || !host_out) return fail(ctx, EKPNP_ERR_INVALID, "bad species or output");
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const void* src = ctx->lat[ctx->cur];
+  if (!ctx->state_q) {
+    HIP_OR_FAIL(ctx, launch_to_q(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1], ctx->stream));
+    src = ctx->lat[ctx->cur ^ 1];
+  }
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  const ChanDesc& d = ctx->desc[channel];
+  const size_t nn = (size_t)d.nx * d.ny;
+  const double ref = species == 0 ? d.rho_ref : (species == 1 ? d.cp_ref : d.cn_ref);
+  const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
+  const char* base = (const char*)src + off * ctx->elem;
+  if (ctx->elem == 8) {
+    HIP_OR_FAIL(ctx, hipMemcpy2D(host_out, nn * 8, base, (size_t)d.plane * 8, nn * 8, 9, hipMemcpyDeviceToHost));
+  } else {
+    std::vector<float> tmp(9 * nn);
+    HIP_OR_FAIL(ctx, hipMemcpy2D(tmp.data(), nn * 4, base, (size_t)d.plane * 4, nn * 4, 9, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < 9 * nn; ++i) host_out[i] = (double)tmp[i];
+  }
+  for (int dd = 0; dd < 9; ++dd)
+    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += W[dd] * ref;
+  return EKPNP_OK;
+}
+
+int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double* host_in) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (species < 0 || species > 2 || !host_in) return fail(ctx, EKPNP_ERR_INVALID, "bad species or input");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  if (ctx->has_pops && !ctx->state_q) {
+    // bring every channel to the unshifted pre-collision form first
+    HIP_OR_FAIL(ctx, launch_to_q(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1], ctx->stream));
+    ctx->cur ^= 1;
+  }
+  ctx->state_q = true;
+  const ChanDesc& d = ctx->desc[channel];
+  const size_t nn = (size_t)d.nx * d.ny;
+  const double ref = species == 0 ? d.rho_ref : (species == 1 ? d.cp_ref : d.cn_ref);
+  const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
+  char* base = (char*)ctx->lat[ctx->cur] + off * ctx->elem;
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->elem == 8) {
+    std::vector<double> tmp(9 * nn);
+    for (int dd = 0; dd < 9; ++dd)
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - W[dd] * ref;
+    HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 8, tmp.data(), nn * 8, nn * 8, 9, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> tmp(9 * nn);
+    for (int dd = 0; dd < 9; ++dd)
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - W[dd] * ref);
+    HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 4, tmp.data(), nn * 4, nn * 4, 9, hipMemcpyHostToDevice));
+  }
+  ctx->has_pops = true;
+  return EKPNP_OK;
+}
+
+int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
+  if (!ctx || !out) return EKPNP_ERR_INVALID;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, launch_reductions(ctx->sb, 1, ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_red, 8 * (size_t)ctx->nchan * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  const long long interval = ctx->snapshot_step >= 0 ? ctx->steps - ctx->snapshot_step : 0;
+  for (int ch = 0; ch < ctx->nchan; ++ch) {
+    double* r = out + 8 * ch;
+    double res = 0.0;
+    if (interval > 0) {
+      res = (r[EKPNP_RED_UX2] > 0.0) ? std::sqrt(r[EKPNP_RED_DUX2] / r[EKPNP_RED_UX2]) / (double)interval
+                                     : (r[EKPNP_RED_DUX2] > 0.0 ? INFINITY : 0.0);
+    }
+    r[EKPNP_RED_RESIDUAL] = res;
+  }
+  ctx->snapshot_step = ctx->steps;
+  return EKPNP_OK;
+}
+
+int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_steps, int* steps_out) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (check_every < 2 || (check_every & 1)) return fail(ctx, EKPNP_ERR_INVALID, "check_every must be even and >= 2");
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
+  std::vector<double> red(8 * (size_t)ctx->nchan);
+  std::vector<int> done(ctx->nchan, -1);
+  int rc = ekpnp_reductions(ctx, red.data());  // snapshot
+  if (rc) return rc;
+  long long start = ctx->steps;
+  int remaining = ctx->nchan;
+  while (remaining > 0 && ctx->steps - start < max_steps) {
+    rc = ekpnp_step(ctx, check_every);
+    if (rc) return rc;
+    rc = ekpnp_reductions(ctx, red.data());
+    if (rc) return rc;
+    for (int ch = 0; ch < ctx->nchan; ++ch) {
+      if (red[8 * ch + EKPNP_RED_NONFINITE] > 0.0) {
+        if (steps_out)
+          for (int k = 0; k < ctx->nchan; ++k) steps_out[k] = done[k];
+        char buf[128];
+        snprintf(buf, sizeof buf, "channel %d: non-finite values after %lld steps", ch, ctx->steps - start);
+        return fail(ctx, EKPNP_ERR_UNSTABLE, buf);
+      }
+      if (done[ch] < 0 && red[8 * ch + EKPNP_RED_RESIDUAL] <= tol) {
+        done[ch] = (int)(ctx->steps - start);
+        --remaining;
+      }
+    }
+  }
+  if (steps_out)
+    for (int k = 0; k < ctx->nchan; ++k) steps_out[k] = done[k];
+  return EKPNP_OK;
+}
+
+int ekpnp_num_channels(const ekpnp_ctx* ctx) { return ctx ? ctx->nchan : 0; }
+long long ekpnp_steps_done(const ekpnp_ctx* ctx) { return ctx ? ctx->steps : 0; }
+size_t ekpnp_device_bytes(const ekpnp_ctx* ctx) { return ctx ? ctx->dev_bytes : 0; }
+
+int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out) {
+  if (!ctx || !ms_out || nsteps <= 0) return EKPNP_ERR_INVALID;
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  if (ctx->state_q) {
+    int rc = ekpnp_step(ctx, 1);
+    if (rc) return rc;
+  }
+  hipEvent_t ev[5];
+  for (int i = 0; i < 5; ++i) HIP_OR_FAIL(ctx, hipEventCreate(&ev[i]));
+  double acc[4] = {0, 0, 0, 0};
+  int rc = EKPNP_OK;
+  for (int it = 0; it < nsteps && rc == EKPNP_OK; ++it) {
+    hipError_t e = hipEventRecord(ev[0], ctx->stream);
+    if (e == hipSuccess) e = enqueue_lbm(ctx, false);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+    if (e == hipSuccess) e = launch_poisson_fwd(ctx->sb, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
+    if (e == hipSuccess) e = launch_poisson_tri(ctx->sb, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[3], ctx->stream);
+    if (e == hipSuccess) e = launch_poisson_inv(ctx->sb, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ev[4], ctx->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
+    if (e != hipSuccess) {
+      rc = fail(ctx, EKPNP_ERR_HIP, std::string("time_stages: ") + hipGetErrorString(e));
+      break;
+    }
+    for (int k = 0; k < 4; ++k) {
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      acc[k] += ms;
+    }
+    ctx->steps++;
+  }
+  if (rc == EKPNP_OK) {
+    for (int k = 0; k < 4; ++k) ms_out[k] = acc[k] / nsteps;
+    hipError_t e = hipEventRecord(ev[0], ctx->stream);
+    long long s0 = ctx->steps;
+    if (e == hipSuccess) rc = ekpnp_step(ctx, nsteps);
+    ctx->steps = s0 + nsteps;
+    if (rc == EKPNP_OK) {
+      e = hipEventRecord(ev[1], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (e != hipSuccess) rc = fail(ctx, EKPNP_ERR_HIP, std::string("time_stages: ") + hipGetErrorString(e));
+      ms_out[4] = ms / nsteps;
+    }
+  }
+  for (int i = 0; i < 5; ++i) (void)hipEventDestroy(ev[i]);
+  return rc;
+}
+
+}  // extern "C"

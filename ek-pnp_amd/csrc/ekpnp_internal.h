@@ -1,0 +1,95 @@
+// ekpnp_internal.h -- device-side descriptors shared by the gfx950 kernels
+// (ekpnp_kernels.hip) and the host side of the C ABI (ekpnp_api.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ekpnp {
+
+// Wall arrays are stored as 6 planes of length total_nx (sum of nx over the
+// channels); channel c owns [wall_off, wall_off + nx) of each plane.
+enum WallPlane { W_PSI_LO = 0, W_PSI_HI = 1, W_CP_LO = 2, W_CP_HI = 3, W_CN_LO = 4, W_CN_HI = 5 };
+
+// Macroscopic output planes (FP64), each of total_nodes elements.
+enum MacroPlane { M_RHO = 0, M_UX = 1, M_UY = 2, M_CP = 3, M_CN = 4, M_COUNT = 5 };
+
+constexpr int LBM_WG = 256;       // nodes per LBM work-group (one node per lane)
+constexpr int FFT_WG = 512;       // lanes of an x-FFT work-group
+constexpr int FFT_MAX_ELEMS = 4096;  // complex values held in LDS by one FFT work-group
+constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 8
+constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
+constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
+constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
+constexpr int MAX_INTERIOR_ROWS = 2048;  // 64 chunks x 32 rows per tridiagonal column
+
+struct ChanDesc {
+  int nx, ny, nk, flags;        // nk = nx/2 + 1 spectral columns
+  long long pop_off;            // first element of this channel's 27 planes
+  long long plane;              // population plane stride (nx*ny rounded up to 64)
+  long long fld_off;            // first element of this channel's [ny][nx] fields
+  long long spec_off;           // first double2 of this channel's [ny][nk] spectrum
+  int wall_off;                 // offset into each wall plane
+  int tw_off;                   // offset of this channel's twiddles e^{-2 pi i j/nx}
+  int pow2;                     // nx is a power of two (Stockham path)
+  int fft_rows;                 // rows per FFT work-group
+  int tri_C, tri_G, tri_L, tile0;  // tridiagonal chunks, modes per WG, rows per chunk
+  // collision constants (double; the FP32 kernel rounds them once)
+  double inv_cfl, inv_cs2;
+  double om_pf, om_mf;          // omega+ dt, omega- dt, fluid
+  double om_pp, om_mp;          // positive ion
+  double om_pn, om_mn;          // negative ion
+  double src_p, src_m;          // 1 - omega+- dt/2 (Guo source split)
+  double dt, half_dt;
+  double rho_ref, cp_ref, cn_ref;  // storage offsets (0 for FP64 storage)
+  double kp, kn;                // mobilities
+  double fcoef, e_app, exf;     // body force
+  double drift_app;             // e_app if ions drift in the applied field else 0
+  double inv2dx, inv2dy;        // central-difference factors
+  double multis, multid;        // moving-wall terms of the top wall
+  double pc_dy2;                // poisson_coef * dy^2
+  double kdy_scale;             // (2 pi / Lx) * dy
+};
+
+struct StepBuffers {
+  const ChanDesc* chans;
+  const int2* lbm_tiles;        // (channel, first node) per LBM work-group
+  int n_lbm_tiles;
+  const int4* fft_tiles;        // (channel, first row, rows, 0) per FFT work-group
+  int n_fft_tiles;
+  int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
+  unsigned fft_lds_pow2;        // dynamic LDS bytes of the Stockham kernels
+  unsigned fft_lds_dft;         // dynamic LDS bytes of the direct-DFT kernels
+  const int2* tri_tiles;        // (channel, first mode) per tridiagonal work-group
+  int n_tri_tiles;
+  int tri_lmax;                 // 16 or 32
+  double* rho_e;                // c+ - c- per node (FP64)
+  double* psi;                  // potential per node (FP64)
+  double* macro;                // M_COUNT planes
+  long long macro_stride;       // = total nodes
+  double* walls;                // 6 wall planes
+  int walls_stride;             // = total nx
+  const double2* tw;            // twiddles
+  double2* spec;                // spectrum [ny][nk] per channel
+  double* ux_prev;              // residual snapshot
+  double* partials;             // reduction partials [n_lbm_tiles][8]
+  double* red_out;              // reductions [n_channels][8]
+  const int* red_first_tile;    // first LBM tile of each channel (n_channels + 1)
+  int n_channels;
+};
+
+// ---- launchers (ekpnp_kernels.hip) ----------------------------------------
+// precision: 0 FP32 deviations, 1 FP64; compute64: FP64 collision arithmetic
+// for FP32 storage
+hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, const StepBuffers& b,
+                      const void* src, void* dst, hipStream_t s);
+hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s);
+hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s);
+hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
+hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* dst, hipStream_t s);
+// pre-collision populations of the next step, written unshifted into dst
+hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, void* dst,
+                       hipStream_t s);
+hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s);
+
+}  // namespace ekpnp

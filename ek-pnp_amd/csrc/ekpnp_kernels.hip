@@ -1,0 +1,911 @@
+// ekpnp_kernels.hip -- gfx950 kernels of the EK-PNP coupled time step.
+//
+// One step is four launches, each covering every channel of the context:
+//   k_lbm       fused pull collide-stream of f (fluid), h (c+), g (c-) with the
+//               wall rules folded in; writes the post-collision state of every
+//               node in place of the next lattice and the FP64 charge density
+//               (replaces gpu_collide_save, gpu_boundary, gpu_stream,
+//               gpu_bc_charge: LBM.cu:331-1024)
+//   k_fft_fwd   per x-row: right-hand side with the wall lifts, real-to-complex
+//               FFT in LDS (replaces odd_extension + forward cuFFT,
+//               poisson.cu:211-263, 190)
+//   k_tridiag   per wavenumber: the wall-normal tridiagonal solve as a chunked
+//               Thomas sweep joined by affine scans in LDS (replaces
+//               gpu_derivative and the y half of the 2-D cuFFT,
+//               poisson.cu:265-283)
+//   k_fft_inv   per x-row: complex-to-real FFT, writes psi (replaces inverse
+//               cuFFT + odd_extract, poisson.cu:196, 285-306)
+// The electric field is never stored: k_lbm differentiates psi on the fly
+// exactly as gpu_efield/gpu_bc do (poisson.cu:145-174).
+//
+// Lattice state between steps ("P'"): at every node the post-collision
+// populations, except that fluid wall nodes hold the reversed pre-collision
+// populations (full-way bounce-back, LBM.cu:816-846).  The next step gathers
+// them (pull streaming with the reference's x and y wrap, LBM.cu:869-905); ion
+// wall nodes instead anti-bounce-back their own values (LBM.cu:915-1023).
+// Right after init_equilibrium or set_populations the lattice holds the
+// pre-collision populations themselves, unshifted ("Q"), and the first step
+// reads them locally (FROM_Q).
+#include <type_traits>
+
+#include "ekpnp_internal.h"
+
+namespace ekpnp {
+
+constexpr double W0 = 4.0 / 9.0, WS = 1.0 / 9.0, WD = 1.0 / 36.0;  // LBM.h:105-107
+constexpr unsigned FLAG_G0_FLUID_U = 0x2u;
+
+// Work-groups are handed to the 8 XCDs round-robin; remap so that each XCD
+// owns one contiguous range of tiles and neighbouring rows share its L2.
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+  const int q = nwg >> 3, r = nwg & 7;
+  const int xcd = wg & 7, slot = wg >> 3;
+  return xcd * q + min(xcd, r) + slot;
+}
+
+// ---------------------------------------------------------------------------
+// E = -grad(psi) by central differences at node (x, y); wall rows copy Ey of
+// the neighbouring row (poisson.cu:145-174).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void efield_at(const double* __restrict__ ps, int nx, int ny, int x,
+                                          int xm, int xp, int y, double inv2dx, double inv2dy,
+                                          double& Ex, double& Ey) {
+  const double* row = ps + (long long)y * nx;
+  Ex = (row[xm] - row[xp]) * inv2dx;
+  int ya = y - 1, yb = y + 1;
+  if (y == 0) { ya = 0; yb = 2; }
+  if (y == ny - 1) { ya = ny - 3; yb = ny - 1; }
+  Ey = (ps[(long long)ya * nx + x] - ps[(long long)yb * nx + x]) * inv2dy;
+}
+
+// Pre-collision populations of node (x, y), 27 values: f 0..8, h 9..17,
+// g 18..26.  FROM_Q: the lattice holds them unshifted.  Otherwise gather the
+// post-collision values of the upstream neighbours (pull streaming).
+template <typename S, typename C, bool FROM_Q>
+__device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plane, int nx, int ny,
+                                         int x, int y, int xm, int xp, C* q) {
+  const long long r0 = (long long)y * nx;
+  if (FROM_Q) {
+#pragma unroll
+    for (int k = 0; k < 27; ++k) q[k] = (C)P[k * plane + r0 + x];
+    return;
+  }
+  const int ym = (y == 0) ? ny - 1 : y - 1;
+  const int yp = (y == ny - 1) ? 0 : y + 1;
+  const long long rm = (long long)ym * nx, rp = (long long)yp * nx;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const S* B = P + (long long)(9 * s) * plane;
+    q[9 * s + 0] = (C)B[0 * plane + r0 + x];
+    q[9 * s + 1] = (C)B[1 * plane + r0 + xm];
+    q[9 * s + 2] = (C)B[2 * plane + rm + x];
+    q[9 * s + 3] = (C)B[3 * plane + r0 + xp];
+    q[9 * s + 4] = (C)B[4 * plane + rp + x];
+    q[9 * s + 5] = (C)B[5 * plane + rm + xm];
+    q[9 * s + 6] = (C)B[6 * plane + rm + xp];
+    q[9 * s + 7] = (C)B[7 * plane + rp + xp];
+    q[9 * s + 8] = (C)B[8 * plane + rp + xm];
+  }
+}
+
+// Ion wall node: the next pre-collision state is the anti-bounce-back of the
+// node's own post-collision state, q[opp(i)] = -p[i] + 2 w_i c_w
+// (LBM.cu:935-943), written here for storage deviations (dcw = c_w - c_ref).
+template <typename S, typename C>
+__device__ __forceinline__ void abb_own(const S* __restrict__ P, long long plane, long long node,
+                                        C dcw, C* q) {
+  const C two_s = C(2.0 * WS) * dcw, two_d = C(2.0 * WD) * dcw;
+  q[0] = -(C)P[0 * plane + node] + C(2.0 * W0) * dcw;
+  q[3] = -(C)P[1 * plane + node] + two_s;
+  q[4] = -(C)P[2 * plane + node] + two_s;
+  q[1] = -(C)P[3 * plane + node] + two_s;
+  q[2] = -(C)P[4 * plane + node] + two_s;
+  q[7] = -(C)P[5 * plane + node] + two_d;
+  q[8] = -(C)P[6 * plane + node] + two_d;
+  q[5] = -(C)P[7 * plane + node] + two_d;
+  q[6] = -(C)P[8 * plane + node] + two_d;
+}
+
+// ---------------------------------------------------------------------------
+// TRT collision in storage-deviation form.  For a pair (i, opp i) with
+// t = (e_i . v)/(cs^2 CFL) the equilibrium splits into
+//   eq+ = w (dX + X (om + t^2/2)),  eq- = w X t,  om = -|v|^2/(2 cs^2)
+// (LBM.cu:468-552, 599-722); the update is
+//   X_i' = X_i - w+ (X+ - eq+) - w- (X- - eq-)   (LBM.cu:775-803).
+// ---------------------------------------------------------------------------
+template <typename C>
+__device__ __forceinline__ void trt_pair(C& a, C& b, C w, C dX, C X, C om, C t, C omp, C omm) {
+  const C eqp = w * (dX + X * (om + C(0.5) * t * t));
+  const C eqm = w * X * t;
+  const C np = C(0.5) * (a + b) - eqp;
+  const C nm = C(0.5) * (a - b) - eqm;
+  a = a - omp * np - omm * nm;
+  b = b - omp * np + omm * nm;
+}
+
+template <typename C>
+__device__ __forceinline__ void collide_ion(C* q, C dX, C X, C vx, C vy, C omp, C omm, C ics,
+                                            C ticf, C rest_om) {
+  const C om = C(-0.5) * (vx * vx + vy * vy) * ics;
+  const C tx = vx * ticf, ty = vy * ticf;
+  const C eq0 = C(W0) * (dX + X * rest_om);
+  q[0] = q[0] - omp * (q[0] - eq0);
+  trt_pair(q[1], q[3], C(WS), dX, X, om, tx, omp, omm);
+  trt_pair(q[2], q[4], C(WS), dX, X, om, ty, omp, omm);
+  trt_pair(q[5], q[7], C(WD), dX, X, om, tx + ty, omp, omm);
+  trt_pair(q[6], q[8], C(WD), dX, X, om, ty - tx, omp, omm);
+}
+
+// Fluid pair with the Guo source split into its even part
+// S+ = w/cs^2 (-u.F + t (c.F)) and odd part S- = w/cs^2 (c.F), weighted by
+// (1 - w+-/2) dt (LBM.cu:556-572, 745-756).
+template <typename C>
+__device__ __forceinline__ void trt_pair_src(C& a, C& b, C w, C dX, C X, C om, C t, C omp, C omm,
+                                             C uF, C cF, C ics, C sp_dt, C sm_dt) {
+  const C eqp = w * (dX + X * (om + C(0.5) * t * t));
+  const C eqm = w * X * t;
+  const C np = C(0.5) * (a + b) - eqp;
+  const C nm = C(0.5) * (a - b) - eqm;
+  const C Sp = w * ics * (t * cF - uF);
+  const C Sm = w * ics * cF;
+  a = a - omp * np - omm * nm + sp_dt * Sp + sm_dt * Sm;
+  b = b - omp * np + omm * nm + sp_dt * Sp - sm_dt * Sm;
+}
+
+template <typename C>
+__device__ __forceinline__ void collide_fluid(C* q, C dr, C rho, C ux, C uy, C Fx, C Fy, C omp,
+                                              C omm, C ics, C icfl, C sp_dt, C sm_dt) {
+  const C ticf = ics * icfl;
+  const C om = C(-0.5) * (ux * ux + uy * uy) * ics;
+  const C tx = ux * ticf, ty = uy * ticf;
+  const C uF = ux * Fx + uy * Fy;
+  const C cFx = Fx * icfl, cFy = Fy * icfl;
+  const C eq0 = C(W0) * (dr + rho * om);
+  q[0] = q[0] - omp * (q[0] - eq0) + sp_dt * (C(W0) * ics * (-uF));
+  trt_pair_src(q[1], q[3], C(WS), dr, rho, om, tx, omp, omm, uF, cFx, ics, sp_dt, sm_dt);
+  trt_pair_src(q[2], q[4], C(WS), dr, rho, om, ty, omp, omm, uF, cFy, ics, sp_dt, sm_dt);
+  trt_pair_src(q[5], q[7], C(WD), dr, rho, om, tx + ty, omp, omm, uF, cFx + cFy, ics, sp_dt, sm_dt);
+  trt_pair_src(q[6], q[8], C(WD), dr, rho, om, ty - tx, omp, omm, uF, cFy - cFx, ics, sp_dt, sm_dt);
+}
+
+template <typename S>
+struct Compute { using T = double; };
+template <>
+struct Compute<float> { using T = float; };
+
+// Force-corrected velocity of a pre-collision state (LBM.cu:382-392).
+template <typename C>
+struct Moments {
+  C dr, rho, dcp, dcn, rhoe, Fx, Fy, ux, uy;
+};
+
+template <typename C>
+__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, double Ex,
+                                              double Ey) {
+  Moments<C> m;
+  m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+  m.rho = C(cd->rho_ref) + m.dr;
+  m.dcp = ((q[9] + q[10]) + (q[11] + q[12])) + ((q[13] + q[14]) + (q[15] + q[16])) + q[17];
+  m.dcn = ((q[18] + q[19]) + (q[20] + q[21])) + ((q[22] + q[23]) + (q[24] + q[25])) + q[26];
+  m.rhoe = (m.dcp - m.dcn) + C(cd->cp_ref - cd->cn_ref);
+  const C jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
+  const C jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
+  m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
+  m.Fy = C(cd->fcoef) * m.rhoe * C(Ey);
+  const C irho = C(1.0) / m.rho;
+  const C icfl = C(cd->inv_cfl), hdt = C(cd->half_dt);
+  m.ux = (jx * icfl + m.Fx * hdt) * irho;
+  m.uy = (jy * icfl + m.Fy * hdt) * irho;
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// k_lbm: one coupled-step LBM update of every node of every channel.
+// ---------------------------------------------------------------------------
+// S: storage type of the populations; C: arithmetic type of the collision.
+template <typename S, typename C, bool FROM_Q, bool MACRO>
+__global__ __launch_bounds__(LBM_WG) void k_lbm(StepBuffers b, const S* __restrict__ src,
+                                                S* __restrict__ dst) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int2 tile = b.lbm_tiles[wg];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= nx * ny) return;
+  const int y = node / nx;
+  const int x = node - y * nx;
+  const long long plane = cd->plane;
+  const S* __restrict__ P = src + cd->pop_off;
+  S* __restrict__ Q = dst + cd->pop_off;
+  const double* __restrict__ ps = b.psi + cd->fld_off;
+  const int xm = (x == 0) ? nx - 1 : x - 1;
+  const int xp = (x == nx - 1) ? 0 : x + 1;
+  const bool bottom = (y == 0), top = (y == ny - 1);
+
+  // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
+  // included (LBM.cu:402-443); evaluated first so its registers are free.
+  C mux = 0, muy = 0;
+  if (bottom) {
+    C m1[27];
+    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, m1);
+    double Ex1, Ey1;
+    efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
+    const Moments<C> mm = moments(m1, cd, Ex1, Ey1);
+    mux = -mm.ux;
+    muy = -mm.uy;
+  }
+
+  C q[27];
+  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, q);
+  if (!FROM_Q && (bottom || top)) {
+    const double* __restrict__ Wl = b.walls + cd->wall_off + x;
+    const long long ws = b.walls_stride;
+    const double cwp = bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
+    const double cwn = bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - cd->cp_ref), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - cd->cn_ref), q + 18);
+  }
+
+  double Ex, Ey;
+  efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
+  const Moments<C> m = moments(q, cd, Ex, Ey);
+  C ux = m.ux, uy = m.uy;
+  if (bottom) {
+    ux = mux;
+    uy = muy;
+  }
+
+  // charge density for the Poisson solve, summed in FP64
+  {
+    double sp = 0.0, sn = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      sp += (double)q[9 + k];
+      sn += (double)q[18 + k];
+    }
+    b.rho_e[cd->fld_off + node] = (sp - sn) + (cd->cp_ref - cd->cn_ref);
+  }
+  if (MACRO) {
+    const long long o = cd->fld_off + node, st = b.macro_stride;
+    b.macro[M_RHO * st + o] = (double)m.rho;
+    b.macro[M_UX * st + o] = (double)ux;
+    b.macro[M_UY * st + o] = (double)uy;
+    b.macro[M_CP * st + o] = (double)(C(cd->cp_ref) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(cd->cn_ref) + m.dcn);
+  }
+
+  const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
+  // ions: drift velocity u + K E (LBM.cu:479-488)
+  {
+    const C Eix = C(Ex + cd->drift_app), Eiy = C(Ey);
+    const C cp = C(cd->cp_ref) + m.dcp, cn = C(cd->cn_ref) + m.dcn;
+    const C kp = C(cd->kp), kn = C(cd->kn);
+    const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
+    const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
+    const C om_p = C(-0.5) * (vpx * vpx + vpy * vpy) * ics;
+    const C om_n = C(-0.5) * (vnx * vnx + vny * vny) * ics;
+    const C om_f = C(-0.5) * (ux * ux + uy * uy) * ics;
+    collide_ion(q + 9, m.dcp, cp, vpx, vpy, C(cd->om_pp), C(cd->om_mp), ics, ticf, om_p);
+    collide_ion(q + 18, m.dcn, cn, vnx, vny, C(cd->om_pn), C(cd->om_mn), ics, ticf,
+                (cd->flags & FLAG_G0_FLUID_U) ? om_f : om_n);
+  }
+  if (bottom || top) {
+    // full-way bounce-back of the pre-collision state, f0 unchanged
+    // (LBM.cu:816-846); moving-wall terms on the top wall only
+    const C ms = top ? C(cd->multis) : C(0), md = top ? C(cd->multid) : C(0);
+    const C f1 = q[1], f2 = q[2], f3 = q[3], f4 = q[4];
+    const C f5 = q[5], f6 = q[6], f7 = q[7], f8 = q[8];
+    q[3] = f1 - ms;
+    q[4] = f2;
+    q[1] = f3 + ms;
+    q[2] = f4;
+    q[7] = f5 - md;
+    q[8] = f6 + md;
+    q[5] = f7 + md;
+    q[6] = f8 - md;
+  } else {
+    collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
+                  C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
+  }
+#pragma unroll
+  for (int k = 0; k < 27; ++k) Q[k * plane + node] = (S)q[k];
+}
+
+// ---------------------------------------------------------------------------
+// k_init_eq: equilibrium populations of the stored fields, unshifted
+// (gpu_init_equilibrium, LBM.cu:195-305).
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(LBM_WG) void k_init_eq(StepBuffers b, S* __restrict__ dst) {
+
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= nx * ny) return;
+  const int y = node / nx, x = node - (node / nx) * nx;
+  const long long plane = cd->plane;
+  const long long o = cd->fld_off + node, st = b.macro_stride;
+  const double rho = b.macro[M_RHO * st + o], ux = b.macro[M_UX * st + o];
+  const double uy = b.macro[M_UY * st + o], cp = b.macro[M_CP * st + o];
+  const double cn = b.macro[M_CN * st + o];
+  const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+  double Ex, Ey;
+  efield_at(b.psi + cd->fld_off, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
+  Ex += cd->drift_app;
+  const double ics = cd->inv_cs2, ticf = cd->inv_cs2 * cd->inv_cfl;
+  const double vel[3][2] = {{ux, uy}, {ux + cd->kp * Ex, uy + cd->kp * Ey},
+                            {ux + cd->kn * Ex, uy + cd->kn * Ey}};
+  const double X[3] = {rho, cp, cn};
+  const double ref[3] = {cd->rho_ref, cd->cp_ref, cd->cn_ref};
+  S* __restrict__ Q = dst + cd->pop_off;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const double vx = vel[s][0], vy = vel[s][1];
+    const double om = -0.5 * (vx * vx + vy * vy) * ics;
+    const double tx = vx * ticf, ty = vy * ticf;
+    const double dX = X[s] - ref[s];
+    const double t[9] = {0.0, tx, ty, -tx, -ty, tx + ty, ty - tx, -(tx + ty), tx - ty};
+#pragma unroll
+    for (int d = 0; d < 9; ++d) {
+      const double w = (d == 0) ? W0 : (d < 5 ? WS : WD);
+      const double eq = w * (dX + X[s] * (om + t[d] * (1.0 + 0.5 * t[d])));
+      Q[(9 * s + d) * plane + node] = (S)eq;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_to_q: pre-collision populations of the next step, written unshifted.
+// ---------------------------------------------------------------------------
+template <typename S>
+__global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restrict__ src,
+                                                 S* __restrict__ dst) {
+  using C = typename Compute<S>::T;
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= nx * ny) return;
+  const int y = node / nx, x = node - (node / nx) * nx;
+  const long long plane = cd->plane;
+  const S* __restrict__ P = src + cd->pop_off;
+  const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+  C q[27];
+  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, q);
+  if (y == 0 || y == ny - 1) {
+    const double* __restrict__ Wl = b.walls + cd->wall_off + x;
+    const long long ws = b.walls_stride;
+    const double cwp = (y == 0) ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
+    const double cwn = (y == 0) ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - cd->cp_ref), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - cd->cn_ref), q + 18);
+  }
+  S* __restrict__ Q = dst + cd->pop_off;
+#pragma unroll
+  for (int k = 0; k < 27; ++k) Q[k * plane + node] = (S)q[k];
+}
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// ---------------------------------------------------------------------------
+// Stockham autosort FFT of `rows` rows of length N (a power of two) held in
+// LDS, in place, with a barrier between the reads and the writes of each
+// stage.  Stage (n, s): butterfly (p, q) reads x[q + s(p + j m)], writes
+// y[q + s(R p + j)], twiddle W_n^p (W_n = e^{SIGN 2 pi i/n}).  One radix-2
+// stage first when log2 N is odd, radix-4 after.  tw[j] = e^{-2 pi i j/nx}.
+// ---------------------------------------------------------------------------
+template <int SIGN>
+__device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ tw,
+                         int lgnx) {
+  int lgn = lgN, lgs = 0;
+  if (lgn & 1) {
+    // radix 2
+    const int m = N >> 1;  // n = N at this stage
+    const int nb = rows * m, lgm = lgn - 1;
+    double2 va[FFT_PER_LANE / 2], vb[FFT_PER_LANE / 2];
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE / 2; ++c) {
+      const int bf = threadIdx.x + c * FFT_WG;
+      if (bf < nb) {
+        const int r = bf >> lgm, p = bf & (m - 1);
+        va[c] = x[(r << lgN) + p];
+        vb[c] = x[(r << lgN) + p + m];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE / 2; ++c) {
+      const int bf = threadIdx.x + c * FFT_WG;
+      if (bf < nb) {
+        const int r = bf >> lgm, p = bf & (m - 1);
+        double2 w = tw[p << (lgnx - lgn)];
+        if (SIGN > 0) w = cconj(w);
+        x[(r << lgN) + 2 * p] = cadd(va[c], vb[c]);
+        x[(r << lgN) + 2 * p + 1] = cmul(csub(va[c], vb[c]), w);
+      }
+    }
+    __syncthreads();
+    lgn -= 1;
+    lgs = 1;
+  }
+  while (lgn >= 2) {
+    const int lgm = lgn - 2, m = 1 << lgm, s = 1 << lgs;
+    const int lgper = lgN - 2;  // butterflies per row = N/4
+    const int nb = rows << lgper;
+    double2 v[FFT_PER_LANE];
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE / 4; ++c) {
+      const int bf = threadIdx.x + c * FFT_WG;
+      if (bf < nb) {
+        const int r = bf >> lgper, j = bf & ((1 << lgper) - 1);
+        const int p = j >> lgs, qq = j & (s - 1);
+        const int base = (r << lgN) + qq + s * p;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[4 * c + u] = x[base + s * m * u];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE / 4; ++c) {
+      const int bf = threadIdx.x + c * FFT_WG;
+      if (bf < nb) {
+        const int r = bf >> lgper, j = bf & ((1 << lgper) - 1);
+        const int p = j >> lgs, qq = j & (s - 1);
+        const int ti = p << (lgnx - lgn);
+        double2 w1 = tw[ti], w2 = tw[2 * ti], w3 = tw[3 * ti];
+        if (SIGN > 0) {
+          w1 = cconj(w1);
+          w2 = cconj(w2);
+          w3 = cconj(w3);
+        }
+        const double2 a = v[4 * c + 0], bb = v[4 * c + 1], cc = v[4 * c + 2], d = v[4 * c + 3];
+        const double2 apc = cadd(a, cc), amc = csub(a, cc);
+        const double2 bpd = cadd(bb, d), bmd = csub(bb, d);
+        // forward: -i (b - d); inverse: +i (b - d)
+        const double2 jb = (SIGN < 0) ? make_double2(bmd.y, -bmd.x) : make_double2(-bmd.y, bmd.x);
+        const int ob = (r << lgN) + qq + s * 4 * p;
+        x[ob + 0 * s] = cadd(apc, bpd);
+        x[ob + 1 * s] = cmul(cadd(amc, jb), w1);
+        x[ob + 2 * s] = cmul(csub(apc, bpd), w2);
+        x[ob + 3 * s] = cmul(csub(amc, jb), w3);
+      }
+    }
+    __syncthreads();
+    lgn -= 2;
+    lgs += 2;
+  }
+}
+
+__device__ __forceinline__ int ilog2(int v) { return 31 - __builtin_clz(v); }
+
+// right-hand side of the lifted wall-normal system, scaled by dy^2
+// (odd_extension, poisson.cu:229-241)
+__device__ __forceinline__ double rhs_at(const double* __restrict__ re, const double* __restrict__ Wl,
+                                         long long ws, const ChanDesc* __restrict__ cd, int x,
+                                         int y) {
+  double v = -cd->pc_dy2 * re[(long long)y * cd->nx + x];
+  if (y == 1) v -= Wl[W_PSI_LO * ws + x];
+  if (y == cd->ny - 2) v -= Wl[W_PSI_HI * ws + x];
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// k_fft_fwd: rows of RHS -> spectrum [y][k], k = 0..nx/2
+// ---------------------------------------------------------------------------
+template <bool POW2>
+__global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base) {
+  extern __shared__ double2 lds[];
+  const int4 t = b.fft_tiles[tile_base + blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
+  const double* __restrict__ re = b.rho_e + cd->fld_off;
+  const double* __restrict__ Wl = b.walls + cd->wall_off;
+  const long long ws = b.walls_stride;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  double2* __restrict__ spec = b.spec + cd->spec_off;
+  if (POW2) {
+    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1;
+    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
+      const int r = e >> lgN, n = e & (N - 1);
+      lds[e] = make_double2(rhs_at(re, Wl, ws, cd, 2 * n, y0 + r), rhs_at(re, Wl, ws, cd, 2 * n + 1, y0 + r));
+    }
+    __syncthreads();
+    stockham<-1>(lds, rows, N, lgN, tw, lgnx);
+    // split the packed transform: X[k] = E[k] + W^k O[k]
+    for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
+      const int r = e / (N + 1), k = e - r * (N + 1);
+      const double2 zk = lds[(r << lgN) + (k & (N - 1))];
+      const double2 zc = cconj(lds[(r << lgN) + ((N - k) & (N - 1))]);
+      const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+      const double2 dd = csub(zk, zc);
+      const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
+      spec[(long long)(y0 + r) * nk + k] = cadd(E, cmul(tw[k], O));
+    }
+  } else {
+    // direct real DFT of one row
+    double* v = reinterpret_cast<double*>(lds);
+    for (int n = threadIdx.x; n < nx; n += FFT_WG) v[n] = rhs_at(re, Wl, ws, cd, n, y0);
+    __syncthreads();
+    for (int k = threadIdx.x; k < nk; k += FFT_WG) {
+      double2 acc = make_double2(0.0, 0.0);
+      int idx = 0;
+      for (int n = 0; n < nx; ++n) {
+        const double2 w = tw[idx];
+        acc.x += v[n] * w.x;
+        acc.y += v[n] * w.y;
+        idx += k;
+        if (idx >= nx) idx -= nx;
+      }
+      spec[(long long)y0 * nk + k] = acc;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_fft_inv: spectrum rows -> psi rows (normalised by 1/nx)
+// ---------------------------------------------------------------------------
+template <bool POW2>
+__global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base) {
+  extern __shared__ double2 lds[];
+  const int4 t = b.fft_tiles[tile_base + blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  const double2* __restrict__ spec = b.spec + cd->spec_off;
+  double* __restrict__ ps = b.psi + cd->fld_off;
+  // Dirichlet rows: psi(0) = voltage, psi(ny-1) = voltage2 (odd_extract,
+  // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
+  {
+    const double* __restrict__ Wl = b.walls + cd->wall_off;
+    const long long ws = b.walls_stride;
+    if (y0 == 1)
+      for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[x] = Wl[W_PSI_LO * ws + x];
+    if (y0 + rows - 1 == cd->ny - 2)
+      for (int x = threadIdx.x; x < nx; x += FFT_WG)
+        ps[(long long)(cd->ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
+  }
+  if (POW2) {
+    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1;
+    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
+      const int r = e >> lgN, k = e & (N - 1);
+      const double2* row = spec + (long long)(y0 + r) * nk;
+      const double2 xk = row[k], xc = cconj(row[N - k]);
+      const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+      const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw[k]));
+      lds[e] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
+    }
+    __syncthreads();
+    stockham<+1>(lds, rows, N, lgN, tw, lgnx);
+    const double invN = 1.0 / (double)N;
+    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
+      const int r = e >> lgN, n = e & (N - 1);
+      const double2 z = lds[e];
+      double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
+      *out = make_double2(z.x * invN, z.y * invN);
+    }
+  } else {
+    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[(long long)y0 * nk + k];
+    __syncthreads();
+    const int kmax = (nx - 1) / 2;  // modes counted twice
+    const bool nyq = (nx & 1) == 0;
+    for (int n = threadIdx.x; n < nx; n += FFT_WG) {
+      double acc = lds[0].x;
+      int idx = 0;
+      for (int k = 1; k <= kmax; ++k) {
+        idx += n;
+        if (idx >= nx) idx -= nx;
+        const double2 w = tw[idx];
+        acc += 2.0 * (lds[k].x * w.x + lds[k].y * w.y);
+      }
+      if (nyq) acc += (n & 1) ? -lds[nx / 2].x : lds[nx / 2].x;
+      ps[(long long)y0 * nx + n] = acc / (double)nx;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_tridiag: for each wavenumber k the system
+//   x_{i-1} + a x_i + x_{i+1} = r_i,  i = 1..M,  x_0 = x_{M+1} = 0,
+//   a = -(2 + (kx dy)^2) = -2 cosh(theta)
+// (the FD-y / spectral-x operator of poisson.cu:274-283, times dy^2).
+// Thomas: c_i = 1/(a - c_{i-1}), y_i = c_i (r_i - y_{i-1}),
+//         x_i = y_i - c_i x_{i+1}.
+// Both sweeps are first-order affine recurrences, so the column is cut into
+// C chunks of L rows (one lane each); every lane sweeps its chunk locally,
+// an LDS scan of the chunk maps (A, B) -> (A A', A B' + B) supplies the
+// missing carry, and a second pass adds carry * running product.  The
+// Thomas coefficient entering a chunk has the closed form
+//   c_i = -sinh(i theta)/sinh((i+1) theta).
+// Lanes: mode m = tid % G, chunk j = tid / G.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double cprime_closed(int i, double kd) {
+  if (kd == 0.0) return -(double)i / (double)(i + 1);
+  const double th = 2.0 * asinh(0.5 * kd);
+  return -exp(-th) * expm1(-2.0 * (double)i * th) / expm1(-2.0 * (double)(i + 1) * th);
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
+  __shared__ double sA[TRI_WG];
+  __shared__ double2 sB[TRI_WG];
+  const int2 t = b.tri_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int G = cd->tri_G, C = cd->tri_C, L = cd->tri_L;
+  const int nk = cd->nk, M = cd->ny - 2;
+  const int tid = threadIdx.x;
+  const int m = tid & (G - 1), j = tid / G;
+  const int k = t.y + m;
+  const bool valid = (k < nk);
+  const int i0 = 1 + j * L;
+  int len = M - j * L;
+  len = len < 0 ? 0 : (len > L ? L : len);
+  double2* __restrict__ col = b.spec + cd->spec_off + (valid ? k : 0);
+  const double kd = (double)k * cd->kdy_scale;
+  const double a = -(2.0 + kd * kd);
+
+  // Straight-line sweeps over LMAX rows; rows at or past `len` are masked
+  // with selects (no branches, so the arrays stay in registers).
+  double cc[LMAX], yr[LMAX], yi[LMAX];
+#pragma unroll
+  for (int ii = 0; ii < LMAX; ++ii) {
+    const int row = min(i0 + ii, M);  // clamped: always a valid row
+    const double2 v = col[(long long)row * nk];
+    yr[ii] = v.x;
+    yi[ii] = v.y;
+  }
+  // local forward sweep with y_{i0-1} = 0
+  double cprev = (i0 > 1 && len > 0) ? cprime_closed(i0 - 1, kd) : 0.0;
+  double A = 1.0, pr = 0.0, pim = 0.0;
+#pragma unroll
+  for (int ii = 0; ii < LMAX; ++ii) {
+    const bool on = ii < len;
+    const double c = 1.0 / (a - cprev);
+    cc[ii] = on ? c : 0.0;
+    cprev = on ? c : cprev;
+    pr = on ? c * (yr[ii] - pr) : pr;
+    pim = on ? c * (yi[ii] - pim) : pim;
+    yr[ii] = pr;
+    yi[ii] = pim;
+    A = on ? -c * A : A;
+  }
+  sA[tid] = A;
+  sB[tid] = make_double2(pr, pim);
+  __syncthreads();
+  // inclusive prefix scan over chunks: S_j = M_j o S_{j-1}
+  for (int d = 1; d < C; d <<= 1) {
+    const bool has = j >= d;
+    double Ap = 1.0;
+    double2 Bp = make_double2(0.0, 0.0);
+    if (has) {
+      Ap = sA[tid - d * G];
+      Bp = sB[tid - d * G];
+    }
+    const double Ac = sA[tid];
+    const double2 Bc = sB[tid];
+    __syncthreads();
+    if (has) {
+      sA[tid] = Ac * Ap;
+      sB[tid] = make_double2(Ac * Bp.x + Bc.x, Ac * Bp.y + Bc.y);
+    }
+    __syncthreads();
+  }
+  const double2 Yin = (j > 0) ? sB[tid - G] : make_double2(0.0, 0.0);
+  __syncthreads();
+  // carry into the forward sweep, then local backward sweep with x_{e+1} = 0
+  // (masked rows have cc = 0: the carry product becomes 0 there, and the
+  // backward recurrence passes x through unchanged only for rows < len)
+  double P = 1.0;
+#pragma unroll
+  for (int ii = 0; ii < LMAX; ++ii) {
+    P *= -cc[ii];
+    yr[ii] += P * Yin.x;
+    yi[ii] += P * Yin.y;
+  }
+  double xr = 0.0, xim = 0.0;
+#pragma unroll
+  for (int ii = LMAX - 1; ii >= 0; --ii) {
+    const bool on = ii < len;
+    const double nxr = yr[ii] - cc[ii] * xr;
+    const double nxi = yi[ii] - cc[ii] * xim;
+    xr = on ? nxr : xr;
+    xim = on ? nxi : xim;
+    yr[ii] = xr;
+    yi[ii] = xim;
+  }
+  sA[tid] = A;
+  sB[tid] = make_double2(xr, xim);
+  __syncthreads();
+  // inclusive suffix scan: T_j = N_j o T_{j+1}
+  for (int d = 1; d < C; d <<= 1) {
+    const bool has = j + d < C;
+    double An = 1.0;
+    double2 Bn = make_double2(0.0, 0.0);
+    if (has) {
+      An = sA[tid + d * G];
+      Bn = sB[tid + d * G];
+    }
+    const double Ac = sA[tid];
+    const double2 Bc = sB[tid];
+    __syncthreads();
+    if (has) {
+      sA[tid] = Ac * An;
+      sB[tid] = make_double2(Ac * Bn.x + Bc.x, Ac * Bn.y + Bc.y);
+    }
+    __syncthreads();
+  }
+  const double2 Xin = (j + 1 < C) ? sB[tid + G] : make_double2(0.0, 0.0);
+  P = 1.0;
+#pragma unroll
+  for (int ii = LMAX - 1; ii >= 0; --ii) {
+    const bool on = ii < len;
+    P = on ? -cc[ii] * P : P;
+    yr[ii] += P * Xin.x;
+    yi[ii] += P * Xin.y;
+  }
+  if (valid) {
+#pragma unroll
+    for (int ii = 0; ii < LMAX; ++ii)
+      if (ii < len) col[(long long)(i0 + ii) * nk] = make_double2(yr[ii], yi[ii]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 reductions per channel: partial sums per LBM tile (wave shuffles +
+// LDS), then one work-group per channel adds its tiles in a fixed order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(LBM_WG) void k_reduce_partial(StepBuffers b, int snapshot) {
+  __shared__ double red[LBM_WG / 64][8];
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node = tile.y + (int)threadIdx.x;
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (node < nx * ny) {
+    const int y = node / nx;
+    const long long o = cd->fld_off + node, st = b.macro_stride;
+    const double rho = b.macro[M_RHO * st + o], ux = b.macro[M_UX * st + o];
+    const double cp = b.macro[M_CP * st + o], cn = b.macro[M_CN * st + o];
+    const double uxp = b.ux_prev[o];
+    v[0] = rho;
+    v[1] = cp;
+    v[2] = cn;
+    if (y > 0 && y < ny - 1) {
+      v[3] = cp - cn;
+      v[4] = (ux - uxp) * (ux - uxp);
+      v[5] = ux * ux;
+    }
+    v[6] = (isfinite(rho) && isfinite(ux) && isfinite(cp) && isfinite(cn)) ? 0.0 : 1.0;
+    if (snapshot) b.ux_prev[o] = ux;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < LBM_WG / 64; ++w) s += red[w][threadIdx.x];
+    b.partials[(long long)blockIdx.x * 8 + threadIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_final(StepBuffers b) {
+  __shared__ double red[4][8];
+  const int c = blockIdx.x;
+  const int t0 = b.red_first_tile[c], t1 = b.red_first_tile[c + 1];
+  double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int t = t0 + (int)threadIdx.x; t < t1; t += 256) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] += b.partials[(long long)t * 8 + k];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 8)
+    b.red_out[c * 8 + threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
+                                     (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename S, typename C>
+static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, const void* src,
+                               void* dst, hipStream_t s) {
+  const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
+  const S* sp = static_cast<const S*>(src);
+  S* dp = static_cast<S*>(dst);
+  if (from_q) {
+    if (macro) hipLaunchKernelGGL((k_lbm<S, C, true, true>), grid, block, 0, s, b, sp, dp);
+    else hipLaunchKernelGGL((k_lbm<S, C, true, false>), grid, block, 0, s, b, sp, dp);
+  } else {
+    if (macro) hipLaunchKernelGGL((k_lbm<S, C, false, true>), grid, block, 0, s, b, sp, dp);
+    else hipLaunchKernelGGL((k_lbm<S, C, false, false>), grid, block, 0, s, b, sp, dp);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, const StepBuffers& b,
+                      const void* src, void* dst, hipStream_t s) {
+  if (precision == 0) {
+    if (compute64) return lbm_dispatch<float, double>(from_q, macro, b, src, dst, s);
+    return lbm_dispatch<float, float>(from_q, macro, b, src, dst, s);
+  }
+  return lbm_dispatch<double, double>(from_q, macro, b, src, dst, s);
+}
+
+// The FFT tile table holds the power-of-two tiles first (n_fft_pow2 of them),
+// then the direct-DFT tiles.
+hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
+  if (b.n_fft_pow2 > 0)
+    hipLaunchKernelGGL((k_fft_fwd<true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  if (b.n_fft_tiles > b.n_fft_pow2)
+    hipLaunchKernelGGL((k_fft_fwd<false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+                       b.fft_lds_dft, s, b, b.n_fft_pow2);
+  return hipGetLastError();
+}
+
+hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
+  if (b.n_fft_pow2 > 0)
+    hipLaunchKernelGGL((k_fft_inv<true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  if (b.n_fft_tiles > b.n_fft_pow2)
+    hipLaunchKernelGGL((k_fft_inv<false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+                       b.fft_lds_dft, s, b, b.n_fft_pow2);
+  return hipGetLastError();
+}
+
+hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s) {
+  if (b.tri_lmax <= 16)
+    hipLaunchKernelGGL((k_tridiag<16>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
+  else
+    hipLaunchKernelGGL((k_tridiag<32>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* dst, hipStream_t s) {
+  const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
+  if (precision == 0) hipLaunchKernelGGL((k_init_eq<float>), grid, block, 0, s, b, static_cast<float*>(dst));
+  else hipLaunchKernelGGL((k_init_eq<double>), grid, block, 0, s, b, static_cast<double*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, void* dst, hipStream_t s) {
+  const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
+  if (precision == 0)
+    hipLaunchKernelGGL((k_to_q<float>), grid, block, 0, s, b, static_cast<const float*>(src),
+                       static_cast<float*>(dst));
+  else
+    hipLaunchKernelGGL((k_to_q<double>), grid, block, 0, s, b, static_cast<const double*>(src),
+                       static_cast<double*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partial, dim3(b.n_lbm_tiles), dim3(LBM_WG), 0, s, b, snapshot);
+  hipLaunchKernelGGL(k_reduce_final, dim3(b.n_channels), dim3(256), 0, s, b);
+  return hipGetLastError();
+}
+
+}  // namespace ekpnp

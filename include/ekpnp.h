@@ -1,0 +1,221 @@
+/*
+ * ekpnp.h -- C ABI of the MI355X (gfx950) EK-PNP coupled time step.
+ *
+ * One opaque context owns every device buffer, the per-channel descriptor
+ * tables, the FFT twiddle tables, one HIP stream and the cached hipGraphs.
+ * A context advances one or many independent channels (periodic in x, walls
+ * at y = 0 and y = ny-1) in lock-step; each channel has its own size and
+ * physics.  All entry points return EKPNP_OK (0) or a negative status; none
+ * calls exit().  A context is not thread-safe; calls on one context are
+ * ordered on its stream.  Host buffers are owned by the caller.
+ *
+ * Entry points and the reference interface each one replaces
+ * (file:line in gyf135/EK-PNP):
+ *   ekpnp_create            main.cu:89-142   (cudaMalloc of populations and
+ *                                             fields, cufftPlan2d, kx/ky tables)
+ *   ekpnp_destroy           main.cu:278-301
+ *   ekpnp_initialization    LBM.cu:95-160    (initialization / gpu_initialization)
+ *   ekpnp_set_fields        LBM.cu:1350-1391 (read_data: macroscopic fields in)
+ *   ekpnp_init_equilibrium  LBM.cu:183-305   (init_equilibrium)
+ *   ekpnp_stream_collide_save LBM.cu:307-329 (collide, boundary, stream, bc_charge)
+ *   ekpnp_fast_poisson      poisson.cu:180-208 (+ efield, poisson.cu:132-174)
+ *   ekpnp_step              main.cu:190-199  (the time loop body, n times)
+ *   ekpnp_get_field         LBM.cu:1272-1312 (the D2H copies of save_data_end)
+ *   ekpnp_get_populations / ekpnp_set_populations
+ *                           f0/f1, h0/h1, temp0/temp1 device arrays, LBM.h:124-126
+ */
+#ifndef EKPNP_H
+#define EKPNP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  EKPNP_OK = 0,
+  EKPNP_ERR_INVALID = -1,     /* bad argument or parameter set              */
+  EKPNP_ERR_HIP = -2,         /* HIP runtime reported an error              */
+  EKPNP_ERR_NOMEM = -3,       /* device or host allocation failed           */
+  EKPNP_ERR_STATE = -4,       /* entry point called out of order            */
+  EKPNP_ERR_UNSUPPORTED = -5, /* shape outside the kernels' limits          */
+  EKPNP_ERR_UNSTABLE = -6     /* NaN / runaway found by the reductions      */
+};
+
+/* ---- population storage ------------------------------------------------ */
+enum {
+  EKPNP_POP_FP32 = 0, /* FP32 deviations from w_i*rho0, w_i*c_bulk         */
+  EKPNP_POP_FP64 = 1  /* FP64 full populations (the reference's storage)   */
+};
+
+/* ---- physics / transcription flags (ekpnp_channel_params.flags) -------- */
+/* Ions drift in E + E_applied; the reference drifts in the solved E only
+ * (LBM.cu:479-488). */
+#define EKPNP_FLAG_ION_DRIFT_APPLIED 0x1u
+/* Negative-ion rest equilibrium uses the fluid |u|^2, as LBM.cu:493 does;
+ * off = the drift velocity, as LBM.cu:242 does at initialisation. */
+#define EKPNP_FLAG_G0_FLUID_U 0x2u
+
+/* ---- fields ------------------------------------------------------------ */
+enum {
+  EKPNP_FIELD_RHO = 0,  /* r      */
+  EKPNP_FIELD_UX = 1,   /* u      */
+  EKPNP_FIELD_UY = 2,   /* v      */
+  EKPNP_FIELD_CPOS = 3, /* c      (positive ion concentration) */
+  EKPNP_FIELD_CNEG = 4, /* T/temp (negative ion concentration) */
+  EKPNP_FIELD_PSI = 5,  /* phi    */
+  EKPNP_FIELD_EX = 6,
+  EKPNP_FIELD_EY = 7,
+  EKPNP_FIELD_COUNT = 8
+};
+
+/* species for ekpnp_get_populations / ekpnp_set_populations */
+enum { EKPNP_SPECIES_F = 0, EKPNP_SPECIES_H = 1, EKPNP_SPECIES_G = 2 };
+
+/* initialisation modes for ekpnp_initialization */
+enum {
+  EKPNP_INIT_COLD_NOISE = 0, /* rho0, u = 0, c+- = c_bulk(1 + noise*U(-1,1))
+                                inside, wall values on the walls, then one
+                                Poisson solve for psi and E                 */
+  EKPNP_INIT_REFERENCE = 1   /* gpu_initialization values (LBM.cu:147-160):
+                                rho0, c+- = 0, psi = psi_lo, u = E = 0      */
+};
+
+/* indices into the per-channel record of ekpnp_reductions */
+enum {
+  EKPNP_RED_MASS = 0,      /* sum rho over all nodes                        */
+  EKPNP_RED_CPOS = 1,      /* sum c+ over all nodes                         */
+  EKPNP_RED_CNEG = 2,      /* sum c- over all nodes                         */
+  EKPNP_RED_CHARGE = 3,    /* sum (c+ - c-) over interior rows              */
+  EKPNP_RED_DUX2 = 4,      /* sum (ux - ux_prev)^2 over interior rows       */
+  EKPNP_RED_UX2 = 5,       /* sum ux^2 over interior rows                   */
+  EKPNP_RED_NONFINITE = 6, /* count of non-finite rho/ux/c values           */
+  EKPNP_RED_RESIDUAL = 7,  /* sqrt(DUX2/UX2)/interval (0 before a snapshot) */
+  EKPNP_RED_COUNT = 8
+};
+
+/*
+ * Per-channel parameters.  General units, as LBM.h:39-112 writes them; the
+ * lattice-unit configurations use dx = dy = dt = cfl = 1, cs2 = 1/3.
+ */
+typedef struct ekpnp_channel_params {
+  int32_t nx, ny;    /* NX (periodic), NY (walls at 0 and NY-1), LBM.h:35-36 */
+  uint32_t flags;    /* EKPNP_FLAG_* */
+  int32_t reserved0;
+  double dx, dy, dt; /* LBM.h:42-43, 51 */
+  double cfl;        /* dt/dx, LBM.h:50 */
+  double cs2;        /* cs_square, LBM.h:52 */
+  double rho0;       /* LBM.h:53 */
+  double nu;         /* LBM.h:65 */
+  double diff_pos;   /* diffu, LBM.h:64 */
+  double diff_neg;   /* D, LBM.h:71 */
+  double mob_pos;    /* K, LBM.h:67 */
+  double mob_neg;    /* Kn, LBM.h:74 */
+  double magic_f;    /* V,  LBM.h:110 */
+  double magic_pos;  /* VC, LBM.h:111 */
+  double magic_neg;  /* VT, LBM.h:112 */
+  double psi_lo;     /* voltage,  LBM.h:58 */
+  double psi_hi;     /* voltage2, LBM.h:60 */
+  double cpos_lo;    /* charge0,  LBM.h:55 */
+  double cpos_hi;    /* charge1,  LBM.h:56 */
+  double cneg_lo;    /* charge0n, LBM.h:81 */
+  double cneg_hi;    /* charge1n, LBM.h:83 */
+  double c_bulk;     /* bulk concentration (chargeinf, LBM.h:54): cold-start
+                        value and the FP32 deviation reference              */
+  double poisson_coef; /* convertCtoCharge/eps (poisson.cu:229-259)         */
+  double force_coef;   /* convertCtoCharge in the body force (LBM.cu:388);
+                          0 reproduces the shipped, zeroed force            */
+  double e_applied;    /* Ext, LBM.h:62 */
+  double exf;          /* LBM.h:47 */
+  double uw;           /* top-wall velocity, LBM.h:46 */
+  double lx;           /* Lx, the x period used by kx (main.cu:120-128)     */
+  double init_noise;   /* relative amplitude of the cold-start noise        */
+} ekpnp_channel_params;
+
+typedef struct ekpnp_options {
+  int32_t precision; /* EKPNP_POP_FP32 or EKPNP_POP_FP64 */
+  int32_t device;    /* HIP device ordinal */
+  int32_t use_graph; /* capture ekpnp_step chunks in hipGraphs */
+  int32_t compute_fp64; /* FP32 storage only: collide in FP64 arithmetic */
+} ekpnp_options;
+
+typedef struct ekpnp_ctx ekpnp_ctx;
+
+/* ---- parameter helpers --------------------------------------------------- */
+/* The reference's compiled-in case (LBM.h:31-112, main.cu:38-44). */
+void ekpnp_reference_params(ekpnp_channel_params* p);
+/* Lattice units: tau -> nu = (tau-1/2)/3, magic 1/4 for all species,
+ * D+- = diff, K+- = +-diff (psi in kT/e), psi walls = zeta,
+ * c+ wall = exp(-zeta), c- wall = exp(zeta), c_bulk = 1, rho0 = 1,
+ * kappa = kappa_h/h with h = (ny-1)/2, poisson_coef = kappa^2/2,
+ * force_coef = poisson_coef, e_applied = e_x, lx = nx, noise 1e-3. */
+int ekpnp_lattice_params(ekpnp_channel_params* p, int nx, int ny, double tau,
+                         double zeta, double kappa_h, double e_x, double diff);
+void ekpnp_default_options(ekpnp_options* o);
+
+/* ---- lifetime ------------------------------------------------------------ */
+int ekpnp_create(const ekpnp_channel_params* channels, int n_channels,
+                 const ekpnp_options* opt, ekpnp_ctx** out);
+void ekpnp_destroy(ekpnp_ctx* ctx);
+const char* ekpnp_last_error(const ekpnp_ctx* ctx);
+/* last error of a failed ekpnp_create (no context exists then) */
+const char* ekpnp_create_error(void);
+
+/* ---- initial state -------------------------------------------------------- */
+int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed);
+/* Per-x wall profiles of one channel (length nx each); the uniform values of
+ * the parameter block are used until this is called.  Any pointer may be
+ * NULL to keep that profile. */
+int ekpnp_set_wall_profile(ekpnp_ctx* ctx, int channel, const double* psi_lo,
+                           const double* psi_hi, const double* cpos_lo,
+                           const double* cpos_hi, const double* cneg_lo,
+                           const double* cneg_hi);
+/* Macroscopic fields of one channel, [ny][nx] each.  psi may be NULL: the
+ * potential is then solved from c+ - c-.  E is always -grad(psi). */
+int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho,
+                     const double* ux, const double* uy, const double* cpos,
+                     const double* cneg, const double* psi);
+int ekpnp_init_equilibrium(ekpnp_ctx* ctx);
+
+/* ---- the coupled step ------------------------------------------------------ */
+int ekpnp_stream_collide_save(ekpnp_ctx* ctx);
+int ekpnp_fast_poisson(ekpnp_ctx* ctx);
+int ekpnp_step(ekpnp_ctx* ctx, int nsteps);
+/* Steps until every channel's residual sqrt(sum (ux-ux_prev)^2 / sum ux^2)
+ * / check_every over interior rows is <= tol, or max_steps.  check_every
+ * must be even.  steps_out[channel] receives the step count at which each
+ * channel first met tol (-1 if it never did); may be NULL. */
+int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every,
+                          int max_steps, int* steps_out);
+
+/* ---- output --------------------------------------------------------------- */
+int ekpnp_get_field(ekpnp_ctx* ctx, int channel, int field, double* host_out);
+/* Pre-collision populations of the next step, reference layout:
+ * [9][ny][nx], index 0 = rest (f0), 1..8 = f1 directions. */
+int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species,
+                          double* host_out);
+int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species,
+                          const double* host_in);
+/* Per-channel FP64 reductions, out[channel*EKPNP_RED_COUNT + EKPNP_RED_*];
+ * also snapshots ux for the next residual. */
+int ekpnp_reductions(ekpnp_ctx* ctx, double* out);
+int ekpnp_synchronize(ekpnp_ctx* ctx);
+
+/* ---- introspection / timing -------------------------------------------------- */
+int ekpnp_num_channels(const ekpnp_ctx* ctx);
+long long ekpnp_steps_done(const ekpnp_ctx* ctx);
+/* Device bytes held by the context. */
+size_t ekpnp_device_bytes(const ekpnp_ctx* ctx);
+/* Times nsteps eager steps with HIP events on the context's stream, per
+ * stage: ms_out[0] LBM, [1] forward x-FFT, [2] wall-normal tridiagonal,
+ * [3] inverse x-FFT, [4] whole step (graph replay of nsteps, per step). */
+int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EKPNP_H */

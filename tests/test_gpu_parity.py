@@ -1,0 +1,153 @@
+"""GPU path (libekpnp.so through the C ABI) against the FP64 CPU oracle.
+
+Every comparison runs both sides from the same seeded input.  FP64 storage
+must agree to rounding (1e-11 relative L2 per field); FP32 deviation storage
+to the 1e-5 relative-L2 bar on u_x, psi, c+ and c-.
+"""
+import numpy as np
+import pytest
+
+import ekpnp
+import oracle as O
+from helpers import FIELDS, assert_parity, field_errors, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+
+
+def _pair(params, precision="fp64", seed=0, mode=ekpnp.COLD_NOISE, method=O.POISSON_XDFT_THOMAS):
+    o = O.Oracle(params, poisson_method=method)
+    o.initialization(mode, seed, 0)
+    o.init_equilibrium()
+    s = ekpnp.Solver([params], precision=precision)
+    s.initialization(mode, seed)
+    s.init_equilibrium()
+    return s, o
+
+
+def _report(tag, errs):
+    print(tag, " ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+
+
+def test_initial_fields_match():
+    p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
+    s, o = _pair(p)
+    assert_parity(s, 0, o, 1e-13, fields=("rho", "cpos", "cneg", "psi", "ex", "ey"), tag="init")
+
+
+@pytest.mark.parametrize("nsteps", [1, 10, 100, 1000])
+def test_fp64_config0_steps(nsteps):
+    # config 0: 64 x 128, tau 0.8, zeta -1, kappa h 8, E_x 1e-4, seed 0
+    p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
+    s, o = _pair(p)
+    s.step(nsteps)
+    o.step(nsteps)
+    assert_parity(s, 0, o, FP64_TOL, tag=f"fp64 n={nsteps}")
+
+
+def test_fp64_populations_match():
+    p = ekpnp.lattice_params(32, 40, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4)
+    s, o = _pair(p, seed=3)
+    s.step(25)
+    o.step(25)
+    for sp in ("f", "h", "g"):
+        e = rel_l2(s.get_populations(0, sp), o.get_populations(sp))
+        print("pops", sp, e, flush=True)
+        assert e < FP64_TOL
+
+
+def test_reference_si_case_dft_path():
+    # the reference's own compiled-in case: NX = 10 (non-power-of-two x rows
+    # use the direct DFT), NY = 101, SI units, g_eq0 quirk on, zero force,
+    # gpu_initialization start (c = 0); 300 steps
+    p = ekpnp.reference_params()
+    s, o = _pair(p, mode=ekpnp.REFERENCE)
+    s.step(300)
+    o.step(300)
+    errs = field_errors(s, 0, o)
+    _report("reference SI", errs)
+    for f in ("rho", "cpos", "cneg", "psi"):
+        assert errs[f] < FP64_TOL, (f, errs[f])
+    # SI velocities are O(1e-12) m/s here: compare on their own scale
+    for f in ("ux", "uy"):
+        a, b = s.get_field(0, f), o.get_field(f)
+        assert np.max(np.abs(a - b)) <= 1e-11 * max(np.max(np.abs(b)), 1e-30) + 1e-30
+
+
+def test_fp64_force_and_flags():
+    # body force on, ions drifting in the applied field, moving top wall
+    p = ekpnp.lattice_params(16, 24, tau=0.9, zeta=-2.0, kappa_h=4.0, e_x=1e-3)
+    p.flags = ekpnp.FLAG_ION_DRIFT_APPLIED | ekpnp.FLAG_G0_FLUID_U
+    p.uw = 1e-3
+    p.exf = 1e-6
+    s, o = _pair(p, seed=7)
+    s.step(200)
+    o.step(200)
+    assert_parity(s, 0, o, 1e-10, tag="flags")
+
+
+def test_stream_collide_save_then_fast_poisson_matches_step():
+    p = ekpnp.lattice_params(32, 48, tau=0.8, zeta=-1.0, kappa_h=5.0, e_x=1e-4)
+    s, o = _pair(p, seed=11)
+    for _ in range(5):
+        s.stream_collide_save()
+        s.fast_poisson()
+    o.step(5)
+    assert_parity(s, 0, o, FP64_TOL, tag="split ops")
+
+
+@pytest.mark.parametrize("nx,ny", [(64, 130), (48, 66), (256, 1026), (32, 2050), (8192, 6)])
+def test_poisson_only(nx, ny):
+    # psi from random c+- with per-x wall potentials, both oracle methods
+    rng = np.random.default_rng(nx * 7919 + ny)
+    p = ekpnp.lattice_params(nx, ny, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
+    shp = (ny, nx)
+    cp = 1 + 0.1 * rng.standard_normal(shp)
+    cn = 1 + 0.1 * rng.standard_normal(shp)
+    lo = -1 + 0.3 * rng.standard_normal(nx)
+    hi = -0.5 + 0.3 * rng.standard_normal(nx)
+    s = ekpnp.Solver([p], precision="fp64")
+    s.set_wall_profile(0, psi_lo=lo, psi_hi=hi)
+    s.set_fields(0, np.ones(shp), np.zeros(shp), np.zeros(shp), cp, cn)
+    o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
+    o.set_wall_profile(psi_lo=lo, psi_hi=hi)
+    o.set_fields(np.ones(shp), np.zeros(shp), np.zeros(shp), cp, cn)
+    e = rel_l2(s.get_field(0, "psi"), o.get_field("psi"))
+    print("poisson", nx, ny, e, flush=True)
+    assert e < 1e-12
+
+
+def test_batched_channels_match_single():
+    # three channels of different size and physics in one context
+    ps = [ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4),
+          ekpnp.lattice_params(32, 64, tau=0.7, zeta=-2.0, kappa_h=4.0, e_x=5e-4),
+          ekpnp.lattice_params(10, 37, tau=0.9, zeta=-0.5, kappa_h=3.0, e_x=1e-5)]
+    s = ekpnp.Solver(ps, precision="fp64")
+    s.initialization(ekpnp.COLD_NOISE, 5)
+    s.init_equilibrium()
+    s.step(60)
+    for ch, p in enumerate(ps):
+        o = O.Oracle(p)
+        o.initialization(ekpnp.COLD_NOISE, 5, ch)
+        o.init_equilibrium()
+        o.step(60)
+        assert_parity(s, ch, o, FP64_TOL, tag=f"batched ch{ch}")
+
+
+@pytest.mark.parametrize("compute64", [False, True])
+def test_fp32_deviation_storage_config0(compute64):
+    p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
+    o = O.Oracle(p)
+    o.initialization(ekpnp.COLD_NOISE, 0, 0)
+    o.init_equilibrium()
+    s = ekpnp.Solver([p], precision="fp32", compute_fp64=compute64)
+    s.initialization(ekpnp.COLD_NOISE, 0)
+    s.init_equilibrium()
+    for n in (3000, 17000):
+        s.step(n)
+        o.step(n)
+        errs = field_errors(s, 0, o)
+        _report(f"fp32 compute64={compute64} n={o.steps}", errs)
+    for f in ("ux", "cpos", "cneg", "psi"):
+        assert errs[f] < 1e-5, (f, errs[f])
