@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--config", default="c1")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=20)
+    ap.add_argument("--compute", choices=("fp32", "fp64"), default="fp64",
+                    help="collision arithmetic for FP32 population storage")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
@@ -90,7 +92,7 @@ def main():
     nodes = sum(p.nx * p.ny for p in params)
     print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
-    s = ekpnp.Solver(params, precision=precision)
+    s = ekpnp.Solver(params, precision=precision, compute_fp64=(args.compute == "fp64"))
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.step(args.warmup)
@@ -122,7 +124,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 populations, f64 Poisson" if precision == "fp32" else "f64",
         "data": "synthetic (seeded cold start: c0 = 1 with U(-1e-3, 1e-3) noise)",
-        "config": {"workload": desc, "nodes": nodes, "precision": precision},
+        "config": {"workload": desc, "nodes": nodes, "precision": precision,
+                   "collision_arithmetic": "f64" if (precision == "fp64" or args.compute == "fp64") else "f32"},
         "hbm_frac_coupled": round(mlups * 1e6 * coupled_b / (HBM_PEAK_GBS * 1e9), 4),
         "stages_ms": {k: round(v, 5) for k, v in st.items()},
         "roofline": {

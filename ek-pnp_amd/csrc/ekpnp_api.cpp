@@ -74,6 +74,7 @@ struct ekpnp_ctx {
   double* d_partials = nullptr;
   double* d_red = nullptr;
   size_t dev_bytes = 0;
+  int max_lds = 65536;
   hipStream_t stream = nullptr;
   StepBuffers sb{};
   // state
@@ -126,6 +127,9 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.nx = p.nx;
   d.ny = p.ny;
   d.nk = p.nx / 2 + 1;
+  d.nk_stride = (d.nk + 7) / 8 * 8;  // spectrum rows start on 128-byte lines
+  d.active = 1;
+  d.swap = 0;
   d.flags = (int)p.flags;
   d.pow2 = (is_pow2(p.nx) && p.nx >= 4) ? 1 : 0;
   const double cs2 = p.cs2, dt = p.dt;
@@ -165,6 +169,9 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.multid = 2.0 * p.rho0 * p.uw / cs2 * (1.0 / 36.0) / p.cfl;
   d.pc_dy2 = p.poisson_coef * p.dy * p.dy;
   d.kdy_scale = 2.0 * M_PI / p.lx * p.dy;
+  d.c_bulk = p.c_bulk;
+  d.inv_vt = p.inv_thermal_voltage;
+  d.pb_omega = p.pb_omega;
   // tridiagonal partition: C chunks of L rows, G = TRI_WG / C modes per WG
   const int M = p.ny - 2;
   int C = pow2_ceil((M + 15) / 16);
@@ -287,6 +294,17 @@ int check_channel(ekpnp_ctx* c, int ch) {
   return EKPNP_OK;
 }
 
+// push the host copy of the channel descriptors (active/swap flags) to the device
+hipError_t sync_desc(ekpnp_ctx* c) {
+  hipError_t e = hipMemcpyAsync(c->d_desc, c->desc.data(), c->nchan * sizeof(ChanDesc), hipMemcpyHostToDevice,
+                                c->stream);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(c->stream);
+}
+
+// lattice index holding channel ch's current state
+int chan_lattice(const ekpnp_ctx* c, int ch) { return c->cur ^ c->desc[ch].swap; }
+
 // psi wall rows of one channel from the wall planes
 hipError_t upload_psi_walls(ekpnp_ctx* c, int ch) {
   const ChanDesc& d = c->desc[ch];
@@ -339,6 +357,9 @@ void ekpnp_reference_params(ekpnp_channel_params* p) {
   p->uw = 0.0;
   p->lx = 1.0e-7;             // LBM.h:40
   p->init_noise = 0.0;
+  p->inv_thermal_voltage = 1.6e-19 / 1.38e-23 / 273.0;  // electron/kB/roomT, LBM.h:87-89
+  p->pb_omega = 0.05;         // LBM.h:91
+  p->pb_iterations = 501.0;   // LBM.cu:116
 }
 
 int ekpnp_lattice_params(ekpnp_channel_params* p, int nx, int ny, double tau, double zeta,
@@ -369,6 +390,9 @@ int ekpnp_lattice_params(ekpnp_channel_params* p, int nx, int ny, double tau, do
   p->uw = 0.0;
   p->lx = (double)nx;
   p->init_noise = 1.0e-3;
+  p->inv_thermal_voltage = 1.0;
+  p->pb_omega = 0.05;
+  p->pb_iterations = 501.0;
   return EKPNP_OK;
 }
 
@@ -377,7 +401,7 @@ void ekpnp_default_options(ekpnp_options* o) {
   o->precision = EKPNP_POP_FP32;
   o->device = 0;
   o->use_graph = 1;
-  o->compute_fp64 = 0;
+  o->compute_fp64 = 1;
 }
 
 const char* ekpnp_create_error(void) { return g_create_error.c_str(); }
@@ -436,7 +460,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     d.tw_off = wall;
     pop += 27 * d.plane;
     fld += pad64(nn);
-    spec += pad64((long long)d.ny * d.nk);
+    spec += pad64((long long)d.ny * d.nk_stride);
     wall += (int)pad64(d.nx);
     c->desc[i] = d;
   }
@@ -454,7 +478,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
   // FFT tiles: power-of-two channels first
-  unsigned lds_pow2 = 16, lds_dft = 16;
+  unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;
   for (int pass = 0; pass < 2; ++pass) {
     for (int i = 0; i < n_channels; ++i) {
       ChanDesc& d = c->desc[i];
@@ -465,6 +489,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
         const int N = d.nx / 2;
         rows = std::max(1, std::min(M, FFT_MAX_ELEMS / N));
         lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(rows * N * sizeof(double2)));
+        lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((rows * N + d.nx / 4) * sizeof(double2)));
       } else {
         lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
       }
@@ -565,7 +590,14 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   }
   c->sb = make_buffers(c);
   c->sb.n_fft_pow2 = n_fft_pow2;
-  c->sb.fft_lds_pow2 = lds_pow2;
+  // rows + quadrant twiddle table in LDS when the device allows that much
+  // dynamic LDS per work-group; otherwise twiddles are read from global memory
+  int max_lds = 65536;
+  if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device) != hipSuccess)
+    max_lds = 65536;
+  c->max_lds = max_lds;
+  c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)max_lds) ? 1 : 0;
+  c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
   c->sb.fft_lds_dft = lds_dft;
   c->sb.tri_lmax = tri_lmax;
   for (int i = 0; i < n_channels; ++i) {
@@ -637,7 +669,7 @@ int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho, const doubl
 
 int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
   if (!ctx) return EKPNP_ERR_INVALID;
-  if (mode != EKPNP_INIT_COLD_NOISE && mode != EKPNP_INIT_REFERENCE)
+  if (mode != EKPNP_INIT_COLD_NOISE && mode != EKPNP_INIT_REFERENCE && mode != EKPNP_INIT_PB_PICARD)
     return fail(ctx, EKPNP_ERR_INVALID, "unknown initialization mode");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   for (int ch = 0; ch < ctx->nchan; ++ch) {
@@ -645,7 +677,7 @@ int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
     const ChanDesc& d = ctx->desc[ch];
     const size_t nn = (size_t)d.nx * d.ny;
     std::vector<double> rho(nn, p.rho0), zero(nn, 0.0), cp(nn), cn(nn);
-    if (mode == EKPNP_INIT_REFERENCE) {
+    if (mode == EKPNP_INIT_REFERENCE || mode == EKPNP_INIT_PB_PICARD) {
       // gpu_initialization, LBM.cu:147-160
       std::vector<double> psi(nn, p.psi_lo);
       int rc = ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), zero.data(),
@@ -664,6 +696,21 @@ int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
       if (rc) return rc;
     }
   }
+  if (mode == EKPNP_INIT_PB_PICARD) {
+    // LBM.cu:106-127: psi_old = psi; iterate {Boltzmann c+-, Poisson, relax};
+    // the ux snapshot buffer holds psi_old (the residual snapshot restarts)
+    int iters = 0;
+    for (const auto& p : ctx->params) iters = std::max(iters, (int)p.pb_iterations);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_ux_prev, ctx->d_psi, ctx->total_nodes * sizeof(double),
+                                    hipMemcpyDeviceToDevice, ctx->stream));
+    for (int it = 0; it < iters; ++it) {
+      HIP_OR_FAIL(ctx, launch_pb_fields(ctx->sb, ctx->stream));
+      HIP_OR_FAIL(ctx, enqueue_poisson(ctx, -1));
+      HIP_OR_FAIL(ctx, launch_pb_relax(ctx->sb, ctx->d_ux_prev, ctx->stream));
+    }
+    HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_ux_prev, 0, ctx->total_nodes * sizeof(double), ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  }
   ctx->has_pops = false;
   ctx->steps = 0;
   ctx->snapshot_step = -1;
@@ -674,6 +721,13 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "init_equilibrium before initialization/set_fields");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  bool reset = false;
+  for (auto& d : ctx->desc) {
+    reset |= (d.swap != 0 || d.active != 1);
+    d.swap = 0;
+    d.active = 1;
+  }
+  if (reset) HIP_OR_FAIL(ctx, sync_desc(ctx));
   HIP_OR_FAIL(ctx, launch_init_equilibrium(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   ctx->state_q = true;
@@ -777,10 +831,12 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
   if (species < 0 || species > 2 || !host_out) return fail(ctx, EKPNP_ERR_INVALID, "bad species or output");
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
-  const void* src = ctx->lat[ctx->cur];
+  const int L = chan_lattice(ctx, channel);
+  const void* src = ctx->lat[L];
   if (!ctx->state_q) {
+    // every channel's pre-collision state lands in the lattice its state is not in
     HIP_OR_FAIL(ctx, launch_to_q(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1], ctx->stream));
-    src = ctx->lat[ctx->cur ^ 1];
+    src = ctx->lat[L ^ 1];
   }
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   const ChanDesc& d = ctx->desc[channel];
@@ -815,7 +871,7 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double
   const size_t nn = (size_t)d.nx * d.ny;
   const double ref = species == 0 ? d.rho_ref : (species == 1 ? d.cp_ref : d.cn_ref);
   const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
-  char* base = (char*)ctx->lat[ctx->cur] + off * ctx->elem;
+  char* base = (char*)ctx->lat[chan_lattice(ctx, channel)] + off * ctx->elem;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   if (ctx->elem == 8) {
     std::vector<double> tmp(9 * nn);
@@ -859,32 +915,64 @@ int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_s
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
   std::vector<double> red(8 * (size_t)ctx->nchan);
   std::vector<int> done(ctx->nchan, -1);
+  // converged channels are frozen (their kernels return at once) until the
+  // call ends; frozen_lat remembers which lattice holds each frozen state
+  std::vector<int> frozen_lat(ctx->nchan, -1);
+  auto thaw = [&]() -> int {
+    bool any = false;
+    for (int ch = 0; ch < ctx->nchan; ++ch) {
+      if (frozen_lat[ch] < 0) continue;
+      ctx->desc[ch].swap = ctx->cur ^ frozen_lat[ch];
+      ctx->desc[ch].active = 1;
+      frozen_lat[ch] = -1;
+      any = true;
+    }
+    if (any) HIP_OR_FAIL(ctx, sync_desc(ctx));
+    if (steps_out)
+      for (int k = 0; k < ctx->nchan; ++k) steps_out[k] = done[k];
+    return EKPNP_OK;
+  };
   int rc = ekpnp_reductions(ctx, red.data());  // snapshot
   if (rc) return rc;
   long long start = ctx->steps;
   int remaining = ctx->nchan;
   while (remaining > 0 && ctx->steps - start < max_steps) {
     rc = ekpnp_step(ctx, check_every);
-    if (rc) return rc;
+    if (rc) {
+      thaw();
+      return rc;
+    }
     rc = ekpnp_reductions(ctx, red.data());
-    if (rc) return rc;
+    if (rc) {
+      thaw();
+      return rc;
+    }
+    bool froze = false;
     for (int ch = 0; ch < ctx->nchan; ++ch) {
       if (red[8 * ch + EKPNP_RED_NONFINITE] > 0.0) {
-        if (steps_out)
-          for (int k = 0; k < ctx->nchan; ++k) steps_out[k] = done[k];
         char buf[128];
         snprintf(buf, sizeof buf, "channel %d: non-finite values after %lld steps", ch, ctx->steps - start);
-        return fail(ctx, EKPNP_ERR_UNSTABLE, buf);
+        std::string msg = buf;
+        thaw();
+        return fail(ctx, EKPNP_ERR_UNSTABLE, msg);
       }
       if (done[ch] < 0 && red[8 * ch + EKPNP_RED_RESIDUAL] <= tol) {
         done[ch] = (int)(ctx->steps - start);
         --remaining;
+        frozen_lat[ch] = chan_lattice(ctx, ch);
+        ctx->desc[ch].active = 0;
+        froze = true;
+      }
+    }
+    if (froze && remaining > 0) {
+      hipError_t e = sync_desc(ctx);
+      if (e != hipSuccess) {
+        thaw();
+        return fail(ctx, EKPNP_ERR_HIP, std::string("sync_desc: ") + hipGetErrorString(e));
       }
     }
   }
-  if (steps_out)
-    for (int k = 0; k < ctx->nchan; ++k) steps_out[k] = done[k];
-  return EKPNP_OK;
+  return thaw();
 }
 
 int ekpnp_num_channels(const ekpnp_ctx* ctx) { return ctx ? ctx->nchan : 0; }

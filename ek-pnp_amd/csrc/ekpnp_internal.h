@@ -25,6 +25,10 @@ constexpr int MAX_INTERIOR_ROWS = 2048;  // 64 chunks x 32 rows per tridiagonal 
 
 struct ChanDesc {
   int nx, ny, nk, flags;        // nk = nx/2 + 1 spectral columns
+  int nk_stride;                // spectrum row stride (nk rounded up to 8)
+  int active;                   // 0: the step kernels skip this channel (frozen)
+  int swap;                     // 1: this channel's state is in the step's dst lattice
+  int pad0;
   long long pop_off;            // first element of this channel's 27 planes
   long long plane;              // population plane stride (nx*ny rounded up to 64)
   long long fld_off;            // first element of this channel's [ny][nx] fields
@@ -49,6 +53,7 @@ struct ChanDesc {
   double multis, multid;        // moving-wall terms of the top wall
   double pc_dy2;                // poisson_coef * dy^2
   double kdy_scale;             // (2 pi / Lx) * dy
+  double c_bulk, inv_vt, pb_omega;  // PB-Picard start
 };
 
 struct StepBuffers {
@@ -60,6 +65,7 @@ struct StepBuffers {
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
   unsigned fft_lds_pow2;        // dynamic LDS bytes of the Stockham kernels
   unsigned fft_lds_dft;         // dynamic LDS bytes of the direct-DFT kernels
+  int fft_tw_lds;               // the power-of-two kernels stage twiddles in LDS
   const int2* tri_tiles;        // (channel, first mode) per tridiagonal work-group
   int n_tri_tiles;
   int tri_lmax;                 // 16 or 32
@@ -70,7 +76,7 @@ struct StepBuffers {
   double* walls;                // 6 wall planes
   int walls_stride;             // = total nx
   const double2* tw;            // twiddles
-  double2* spec;                // spectrum [ny][nk] per channel
+  double2* spec;                // spectrum [ny][nk_stride] per channel
   double* ux_prev;              // residual snapshot
   double* partials;             // reduction partials [n_lbm_tiles][8]
   double* red_out;              // reductions [n_channels][8]
@@ -91,5 +97,9 @@ hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* ds
 hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, void* dst,
                        hipStream_t s);
 hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s);
+// PB-Picard start: Boltzmann concentrations from psi, then under-relaxation
+// of psi against psi_old (psi_old is updated to the relaxed value)
+hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s);
+hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s);
 
 }  // namespace ekpnp

@@ -33,6 +33,46 @@
 namespace ekpnp {
 
 constexpr double W0 = 4.0 / 9.0, WS = 1.0 / 9.0, WD = 1.0 / 36.0;  // LBM.h:105-107
+
+// Build-time tuning of k_lbm (defaults chosen by measurement on MI355X, where
+// non-temporal loads and stores of the populations raised the kernel from
+// 5.0 to 5.7 TB/s at 4096 x 1024):
+//   EKPNP_NT_STORE  1: post-collision populations are written with
+//                      non-temporal stores (streamed past the caches)
+//   EKPNP_NT_LOAD   1: the gather reads are non-temporal as well
+//   EKPNP_LBM_WAVES N: ask for at least N waves per SIMD (register budget)
+#ifndef EKPNP_NT_STORE
+#define EKPNP_NT_STORE 1
+#endif
+#ifndef EKPNP_NT_LOAD
+#define EKPNP_NT_LOAD 1
+#endif
+#ifndef EKPNP_LBM_WAVES
+#define EKPNP_LBM_WAVES 0
+#endif
+#if EKPNP_LBM_WAVES > 0
+#define LBM_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(EKPNP_LBM_WAVES)))
+#else
+#define LBM_WAVES_ATTR
+#endif
+
+template <typename S>
+__device__ __forceinline__ S ld_pop(const S* p) {
+#if EKPNP_NT_LOAD
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
+template <typename S>
+__device__ __forceinline__ void st_pop(S* p, S v) {
+#if EKPNP_NT_STORE
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
 constexpr unsigned FLAG_G0_FLUID_U = 0x2u;
 
 // Work-groups are handed to the 8 XCDs round-robin; remap so that each XCD
@@ -67,7 +107,7 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
   const long long r0 = (long long)y * nx;
   if (FROM_Q) {
 #pragma unroll
-    for (int k = 0; k < 27; ++k) q[k] = (C)P[k * plane + r0 + x];
+    for (int k = 0; k < 27; ++k) q[k] = (C)ld_pop(P + k * plane + r0 + x);
     return;
   }
   const int ym = (y == 0) ? ny - 1 : y - 1;
@@ -76,15 +116,15 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const S* B = P + (long long)(9 * s) * plane;
-    q[9 * s + 0] = (C)B[0 * plane + r0 + x];
-    q[9 * s + 1] = (C)B[1 * plane + r0 + xm];
-    q[9 * s + 2] = (C)B[2 * plane + rm + x];
-    q[9 * s + 3] = (C)B[3 * plane + r0 + xp];
-    q[9 * s + 4] = (C)B[4 * plane + rp + x];
-    q[9 * s + 5] = (C)B[5 * plane + rm + xm];
-    q[9 * s + 6] = (C)B[6 * plane + rm + xp];
-    q[9 * s + 7] = (C)B[7 * plane + rp + xp];
-    q[9 * s + 8] = (C)B[8 * plane + rp + xm];
+    q[9 * s + 0] = (C)ld_pop(B + 0 * plane + r0 + x);
+    q[9 * s + 1] = (C)ld_pop(B + 1 * plane + r0 + xm);
+    q[9 * s + 2] = (C)ld_pop(B + 2 * plane + rm + x);
+    q[9 * s + 3] = (C)ld_pop(B + 3 * plane + r0 + xp);
+    q[9 * s + 4] = (C)ld_pop(B + 4 * plane + rp + x);
+    q[9 * s + 5] = (C)ld_pop(B + 5 * plane + rm + xm);
+    q[9 * s + 6] = (C)ld_pop(B + 6 * plane + rm + xp);
+    q[9 * s + 7] = (C)ld_pop(B + 7 * plane + rp + xp);
+    q[9 * s + 8] = (C)ld_pop(B + 8 * plane + rp + xm);
   }
 }
 
@@ -204,19 +244,21 @@ __device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __rest
 // ---------------------------------------------------------------------------
 // S: storage type of the populations; C: arithmetic type of the collision.
 template <typename S, typename C, bool FROM_Q, bool MACRO>
-__global__ __launch_bounds__(LBM_WG) void k_lbm(StepBuffers b, const S* __restrict__ src,
+__global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, const S* __restrict__ src,
                                                 S* __restrict__ dst) {
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
   const int2 tile = b.lbm_tiles[wg];
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
   const int node = tile.y + (int)threadIdx.x;
   if (node >= nx * ny) return;
   const int y = node / nx;
   const int x = node - y * nx;
   const long long plane = cd->plane;
-  const S* __restrict__ P = src + cd->pop_off;
-  S* __restrict__ Q = dst + cd->pop_off;
+  // a channel frozen for an odd number of steps holds its state in the other lattice
+  const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
+  S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
   const double* __restrict__ ps = b.psi + cd->fld_off;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
@@ -308,7 +350,7 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm(StepBuffers b, const S* __restri
                   C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
   }
 #pragma unroll
-  for (int k = 0; k < 27; ++k) Q[k * plane + node] = (S)q[k];
+  for (int k = 0; k < 27; ++k) st_pop(Q + k * plane + node, (S)q[k]);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +411,7 @@ __global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restr
   if (node >= nx * ny) return;
   const int y = node / nx, x = node - (node / nx) * nx;
   const long long plane = cd->plane;
-  const S* __restrict__ P = src + cd->pop_off;
+  const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
   C q[27];
   load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, q);
@@ -381,7 +423,7 @@ __global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restr
     abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - cd->cp_ref), q + 9);
     abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - cd->cn_ref), q + 18);
   }
-  S* __restrict__ Q = dst + cd->pop_off;
+  S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
 #pragma unroll
   for (int k = 0; k < 27; ++k) Q[k * plane + node] = (S)q[k];
 }
@@ -397,15 +439,30 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
 // ---------------------------------------------------------------------------
+// Twiddles: LDS holds e^{-2 pi i j/nx} for j < nx/4 (one quadrant); the rest
+// of [0, nx/2] follows from e^{-2 pi i (j + nx/4)/nx} = -i e^{-2 pi i j/nx}.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 tw_half(const double2* __restrict__ q, int j, int quarter) {
+  if (j < quarter) return q[j];
+  if (j < 2 * quarter) {
+    const double2 w = q[j - quarter];
+    return make_double2(w.y, -w.x);
+  }
+  return make_double2(-1.0, 0.0);  // j == nx/2
+}
+
+// ---------------------------------------------------------------------------
 // Stockham autosort FFT of `rows` rows of length N (a power of two) held in
 // LDS, in place, with a barrier between the reads and the writes of each
 // stage.  Stage (n, s): butterfly (p, q) reads x[q + s(p + j m)], writes
 // y[q + s(R p + j)], twiddle W_n^p (W_n = e^{SIGN 2 pi i/n}).  One radix-2
-// stage first when log2 N is odd, radix-4 after.  tw[j] = e^{-2 pi i j/nx}.
+// stage first when log2 N is odd, radix-4 after.  twq is the LDS quadrant
+// table of e^{-2 pi i j/nx}; W^2p and W^3p are formed by multiplication.
 // ---------------------------------------------------------------------------
 template <int SIGN>
-__device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ tw,
+__device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ twq,
                          int lgnx) {
+  const int quarter = 1 << (lgnx - 2);
   int lgn = lgN, lgs = 0;
   if (lgn & 1) {
     // radix 2
@@ -427,7 +484,7 @@ __device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __
       const int bf = threadIdx.x + c * FFT_WG;
       if (bf < nb) {
         const int r = bf >> lgm, p = bf & (m - 1);
-        double2 w = tw[p << (lgnx - lgn)];
+        double2 w = tw_half(twq, p << (lgnx - lgn), quarter);
         if (SIGN > 0) w = cconj(w);
         x[(r << lgN) + 2 * p] = cadd(va[c], vb[c]);
         x[(r << lgN) + 2 * p + 1] = cmul(csub(va[c], vb[c]), w);
@@ -460,13 +517,10 @@ __device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __
       if (bf < nb) {
         const int r = bf >> lgper, j = bf & ((1 << lgper) - 1);
         const int p = j >> lgs, qq = j & (s - 1);
-        const int ti = p << (lgnx - lgn);
-        double2 w1 = tw[ti], w2 = tw[2 * ti], w3 = tw[3 * ti];
-        if (SIGN > 0) {
-          w1 = cconj(w1);
-          w2 = cconj(w2);
-          w3 = cconj(w3);
-        }
+        // W_n^p with p < n/4 lies in the first quadrant of the table
+        double2 w1 = twq[p << (lgnx - lgn)];
+        if (SIGN > 0) w1 = cconj(w1);
+        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
         const double2 a = v[4 * c + 0], bb = v[4 * c + 1], cc = v[4 * c + 2], d = v[4 * c + 3];
         const double2 apc = cadd(a, cc), amc = csub(a, cc);
         const double2 bpd = cadd(bb, d), bmd = csub(bb, d);
@@ -501,11 +555,13 @@ __device__ __forceinline__ double rhs_at(const double* __restrict__ re, const do
 // ---------------------------------------------------------------------------
 // k_fft_fwd: rows of RHS -> spectrum [y][k], k = 0..nx/2
 // ---------------------------------------------------------------------------
-template <bool POW2>
+// TWL: stage the quadrant twiddle table in LDS (else read it from global)
+template <bool POW2, bool TWL>
 __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base) {
   extern __shared__ double2 lds[];
   const int4 t = b.fft_tiles[tile_base + blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
+  if (!cd->active) return;
   const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
   const double* __restrict__ re = b.rho_e + cd->fld_off;
   const double* __restrict__ Wl = b.walls + cd->wall_off;
@@ -513,13 +569,29 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
   const double2* __restrict__ tw = b.tw + cd->tw_off;
   double2* __restrict__ spec = b.spec + cd->spec_off;
   if (POW2) {
-    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1;
+    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
+    double2* twl = lds + rows * N;  // quadrant twiddle table after the rows
+    if (TWL)
+      for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
+    const double2* twq = TWL ? twl : tw;
+    const double2* r2 = reinterpret_cast<const double2*>(re);
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-      const int r = e >> lgN, n = e & (N - 1);
-      lds[e] = make_double2(rhs_at(re, Wl, ws, cd, 2 * n, y0 + r), rhs_at(re, Wl, ws, cd, 2 * n + 1, y0 + r));
+      const int r = e >> lgN, n = e & (N - 1), y = y0 + r;
+      // x[2n], x[2n+1] of the row as one 16-byte load (rows start 16-byte aligned)
+      const double2 q = r2[(((long long)y * nx) >> 1) + n];
+      double v0 = -cd->pc_dy2 * q.x, v1 = -cd->pc_dy2 * q.y;
+      if (y == 1) {
+        v0 -= Wl[W_PSI_LO * ws + 2 * n];
+        v1 -= Wl[W_PSI_LO * ws + 2 * n + 1];
+      }
+      if (y == cd->ny - 2) {
+        v0 -= Wl[W_PSI_HI * ws + 2 * n];
+        v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
+      }
+      lds[e] = make_double2(v0, v1);
     }
     __syncthreads();
-    stockham<-1>(lds, rows, N, lgN, tw, lgnx);
+    stockham<-1>(lds, rows, N, lgN, twq, lgnx);
     // split the packed transform: X[k] = E[k] + W^k O[k]
     for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
       const int r = e / (N + 1), k = e - r * (N + 1);
@@ -528,7 +600,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
       const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
       const double2 dd = csub(zk, zc);
       const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
-      spec[(long long)(y0 + r) * nk + k] = cadd(E, cmul(tw[k], O));
+      spec[(long long)(y0 + r) * cd->nk_stride + k] = cadd(E, cmul(tw_half(twq, k, quarter), O));
     }
   } else {
     // direct real DFT of one row
@@ -545,7 +617,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
         idx += k;
         if (idx >= nx) idx -= nx;
       }
-      spec[(long long)y0 * nk + k] = acc;
+      spec[(long long)y0 * cd->nk_stride + k] = acc;
     }
   }
 }
@@ -553,11 +625,12 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
 // ---------------------------------------------------------------------------
 // k_fft_inv: spectrum rows -> psi rows (normalised by 1/nx)
 // ---------------------------------------------------------------------------
-template <bool POW2>
+template <bool POW2, bool TWL>
 __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base) {
   extern __shared__ double2 lds[];
   const int4 t = b.fft_tiles[tile_base + blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
+  if (!cd->active) return;
   const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
   const double2* __restrict__ tw = b.tw + cd->tw_off;
   const double2* __restrict__ spec = b.spec + cd->spec_off;
@@ -574,17 +647,23 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
         ps[(long long)(cd->ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
   }
   if (POW2) {
-    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1;
+    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
+    double2* twl = lds + rows * N;
+    if (TWL) {
+      for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
+      __syncthreads();
+    }
+    const double2* twq = TWL ? twl : tw;
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
       const int r = e >> lgN, k = e & (N - 1);
-      const double2* row = spec + (long long)(y0 + r) * nk;
+      const double2* row = spec + (long long)(y0 + r) * cd->nk_stride;
       const double2 xk = row[k], xc = cconj(row[N - k]);
       const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
-      const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw[k]));
+      const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw_half(twq, k, quarter)));
       lds[e] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
     }
     __syncthreads();
-    stockham<+1>(lds, rows, N, lgN, tw, lgnx);
+    stockham<+1>(lds, rows, N, lgN, twq, lgnx);
     const double invN = 1.0 / (double)N;
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
       const int r = e >> lgN, n = e & (N - 1);
@@ -593,7 +672,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
       *out = make_double2(z.x * invN, z.y * invN);
     }
   } else {
-    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[(long long)y0 * nk + k];
+    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[(long long)y0 * cd->nk_stride + k];
     __syncthreads();
     const int kmax = (nx - 1) / 2;  // modes counted twice
     const bool nyq = (nx & 1) == 0;
@@ -639,8 +718,9 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   __shared__ double2 sB[TRI_WG];
   const int2 t = b.tri_tiles[blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
+  if (!cd->active) return;
   const int G = cd->tri_G, C = cd->tri_C, L = cd->tri_L;
-  const int nk = cd->nk, M = cd->ny - 2;
+  const int nk = cd->nk, ks = cd->nk_stride, M = cd->ny - 2;
   const int tid = threadIdx.x;
   const int m = tid & (G - 1), j = tid / G;
   const int k = t.y + m;
@@ -658,7 +738,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
 #pragma unroll
   for (int ii = 0; ii < LMAX; ++ii) {
     const int row = min(i0 + ii, M);  // clamped: always a valid row
-    const double2 v = col[(long long)row * nk];
+    const double2 v = col[(long long)row * ks];
     yr[ii] = v.x;
     yi[ii] = v.y;
   }
@@ -754,7 +834,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   if (valid) {
 #pragma unroll
     for (int ii = 0; ii < LMAX; ++ii)
-      if (ii < len) col[(long long)(i0 + ii) * nk] = make_double2(yr[ii], yi[ii]);
+      if (ii < len) col[(long long)(i0 + ii) * ks] = make_double2(yr[ii], yi[ii]);
   }
 }
 
@@ -829,6 +909,35 @@ __global__ __launch_bounds__(256) void k_reduce_final(StepBuffers b) {
 }
 
 // ---------------------------------------------------------------------------
+// PB-Picard start (LBM.cu:106-145): gpu_PBE and gpu_PBE_phi, all nodes
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(LBM_WG) void k_pb_fields(StepBuffers b) {
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= cd->nx * cd->ny) return;
+  const long long o = cd->fld_off + node, st = b.macro_stride;
+  const double psi = b.psi[o];
+  const double cp = cd->c_bulk * exp(-cd->inv_vt * psi);  // LBM.cu:143
+  const double cn = cd->c_bulk * exp(cd->inv_vt * psi);   // LBM.cu:144
+  b.macro[M_CP * st + o] = cp;
+  b.macro[M_CN * st + o] = cn;
+  b.rho_e[o] = cp - cn;
+}
+
+__global__ __launch_bounds__(LBM_WG) void k_pb_relax(StepBuffers b, double* __restrict__ psi_old) {
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= cd->nx * cd->ny) return;
+  const long long o = cd->fld_off + node;
+  const double w = cd->pb_omega;
+  const double v = w * b.psi[o] + (1.0 - w) * psi_old[o];  // LBM.cu:136
+  b.psi[o] = v;
+  psi_old[o] = v;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename S, typename C>
@@ -859,19 +968,27 @@ hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, co
 // The FFT tile table holds the power-of-two tiles first (n_fft_pow2 of them),
 // then the direct-DFT tiles.
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0)
-    hipLaunchKernelGGL((k_fft_fwd<true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  if (b.n_fft_pow2 > 0) {
+    if (b.fft_tw_lds)
+      hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+    else
+      hipLaunchKernelGGL((k_fft_fwd<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  }
   if (b.n_fft_tiles > b.n_fft_pow2)
-    hipLaunchKernelGGL((k_fft_fwd<false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+    hipLaunchKernelGGL((k_fft_fwd<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
                        b.fft_lds_dft, s, b, b.n_fft_pow2);
   return hipGetLastError();
 }
 
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0)
-    hipLaunchKernelGGL((k_fft_inv<true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  if (b.n_fft_pow2 > 0) {
+    if (b.fft_tw_lds)
+      hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+    else
+      hipLaunchKernelGGL((k_fft_inv<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+  }
   if (b.n_fft_tiles > b.n_fft_pow2)
-    hipLaunchKernelGGL((k_fft_inv<false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+    hipLaunchKernelGGL((k_fft_inv<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
                        b.fft_lds_dft, s, b, b.n_fft_pow2);
   return hipGetLastError();
 }
@@ -899,6 +1016,16 @@ hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, voi
   else
     hipLaunchKernelGGL((k_to_q<double>), grid, block, 0, s, b, static_cast<const double*>(src),
                        static_cast<double*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s) {
+  hipLaunchKernelGGL(k_pb_fields, dim3(b.n_lbm_tiles), dim3(LBM_WG), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s) {
+  hipLaunchKernelGGL(k_pb_relax, dim3(b.n_lbm_tiles), dim3(LBM_WG), 0, s, b, psi_old);
   return hipGetLastError();
 }
 

@@ -26,14 +26,15 @@ from typing import Iterable, Optional, Sequence
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libekpnp.so")
+# EKPNP_LIB selects another build of the same library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("EKPNP_LIB", os.path.join(os.path.dirname(_HERE), "libekpnp.so"))
 
 # status codes (include/ekpnp.h)
 OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED, ERR_UNSTABLE = 0, -1, -2, -3, -4, -5, -6
 POP_FP32, POP_FP64 = 0, 1
 FLAG_ION_DRIFT_APPLIED = 0x1
 FLAG_G0_FLUID_U = 0x2
-COLD_NOISE, REFERENCE = 0, 1
+COLD_NOISE, REFERENCE, PB_PICARD = 0, 1, 2
 FIELDS = {"rho": 0, "ux": 1, "uy": 2, "cpos": 3, "cneg": 4, "psi": 5, "ex": 6, "ey": 7}
 SPECIES = {"f": 0, "h": 1, "g": 2}
 RED_NAMES = ("mass", "cpos", "cneg", "charge", "dux2", "ux2", "nonfinite", "residual")
@@ -42,6 +43,7 @@ _DOUBLE_FIELDS = (
     "dx", "dy", "dt", "cfl", "cs2", "rho0", "nu", "diff_pos", "diff_neg", "mob_pos", "mob_neg",
     "magic_f", "magic_pos", "magic_neg", "psi_lo", "psi_hi", "cpos_lo", "cpos_hi", "cneg_lo",
     "cneg_hi", "c_bulk", "poisson_coef", "force_coef", "e_applied", "exf", "uw", "lx", "init_noise",
+    "inv_thermal_voltage", "pb_omega", "pb_iterations",
 )
 
 
@@ -150,6 +152,9 @@ def reference_params() -> ChannelParams:
     p.exf = p.uw = 0.0                           # LBM.h:46-47
     p.lx = 1.0e-7                                # LBM.h:40
     p.init_noise = 0.0
+    p.inv_thermal_voltage = 1.6e-19 / 1.38e-23 / 273.0  # electron/kB/roomT, LBM.h:87-89
+    p.pb_omega = 0.05                            # LBM.h:91
+    p.pb_iterations = 501.0                      # LBM.cu:116
     return p
 
 
@@ -186,6 +191,9 @@ def lattice_params(nx: int, ny: int, tau: float = 0.8, zeta: float = -1.0, kappa
     p.exf = p.uw = 0.0
     p.lx = float(nx)
     p.init_noise = 1.0e-3
+    p.inv_thermal_voltage = 1.0
+    p.pb_omega = 0.05
+    p.pb_iterations = 501.0
     return p
 
 
@@ -328,6 +336,6 @@ class Solver:
 
 __all__ = [
     "ChannelParams", "Options", "Solver", "EkpnpError", "lib", "reference_params", "lattice_params",
-    "FIELDS", "SPECIES", "RED_NAMES", "COLD_NOISE", "REFERENCE", "POP_FP32", "POP_FP64",
+    "FIELDS", "SPECIES", "RED_NAMES", "COLD_NOISE", "REFERENCE", "PB_PICARD", "POP_FP32", "POP_FP64",
     "FLAG_ION_DRIFT_APPLIED", "FLAG_G0_FLUID_U", "SIGNATURES", "LIB_PATH",
 ]

@@ -1,0 +1,239 @@
+"""Warm-start surrogate: channel parameters -> wall-normal steady profiles.
+
+The network (include/ekpnp_mlp.h) maps four features of a channel
+(zeta, kappa h, E_x, n_y) to psi, c+, c- and u_x at N_PTS points spread
+uniformly over the wall-normal coordinate y in [0, ny-1].  It is trained and
+evaluated by the library's BF16 MFMA kernels; this module only prepares
+features/targets, resamples profiles and drives training.
+
+Targets are normalised so that every output is O(1) across the parameter
+range:
+    psi / zeta,   log(c+) / (-zeta),   log(c-) / zeta,
+    u_x nu (poisson_coef / force_coef) / (E_x (-zeta))
+(for Boltzmann ions and a Helmholtz-Smoluchowski flow all four are close to
+psi/zeta shifted to vanish at the wall for u_x).  The reference has no
+counterpart; its only warm start is the PB-Picard iteration (LBM.cu:106-127).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Sequence
+
+import numpy as np
+
+from . import ChannelParams, EkpnpError, lib
+
+N_PTS = 256
+N_FIELDS = 4          # psi, c+, c-, u_x
+N_OUT = N_PTS * N_FIELDS
+N_IN = 4              # zeta, kappa h, E_x, n_y
+
+
+class MLPConfig(ctypes.Structure):
+    _fields_ = [("n_in", ctypes.c_int32), ("n_hidden", ctypes.c_int32), ("n_layers", ctypes.c_int32),
+                ("n_out", ctypes.c_int32), ("max_batch", ctypes.c_int32), ("device", ctypes.c_int32),
+                ("seed", ctypes.c_uint64)]
+
+
+_VP = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+MLP_SIGNATURES = [
+    ("ekpnp_mlp_create", ctypes.c_int, [ctypes.POINTER(MLPConfig), ctypes.POINTER(_VP)]),
+    ("ekpnp_mlp_destroy", None, [_VP]),
+    ("ekpnp_mlp_last_error", ctypes.c_char_p, [_VP]),
+    ("ekpnp_mlp_num_params", ctypes.c_longlong, [_VP]),
+    ("ekpnp_mlp_get_params", ctypes.c_int, [_VP, _FP]),
+    ("ekpnp_mlp_set_params", ctypes.c_int, [_VP, _FP]),
+    ("ekpnp_mlp_forward", ctypes.c_int, [_VP, _FP, ctypes.c_int, _FP]),
+    ("ekpnp_mlp_loss_grad", ctypes.c_int, [_VP, _FP, _FP, ctypes.c_int, _FP, _FP]),
+    ("ekpnp_mlp_train_step", ctypes.c_int, [_VP, _FP, _FP, ctypes.c_int, ctypes.c_float, _FP]),
+    ("ekpnp_mlp_set_adam", ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+]
+
+_bound = False
+
+
+def _lib():
+    global _bound
+    h = lib()
+    if not _bound:
+        for name, res, args in MLP_SIGNATURES:
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _bound = True
+    return h
+
+
+def _f32(a, shape=None):
+    arr = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        arr = arr.reshape(shape)
+    return arr
+
+
+def _fp(a):
+    return a.ctypes.data_as(_FP)
+
+
+class MLP:
+    """GPU multilayer perceptron (BF16 MFMA, FP32 accumulation, Adam)."""
+
+    def __init__(self, n_in=N_IN, n_hidden=256, n_layers=4, n_out=N_OUT, max_batch=512, seed=0, device=0):
+        L = _lib()
+        self.cfg = MLPConfig(n_in, n_hidden, n_layers, n_out, max_batch, device, seed)
+        h = _VP()
+        rc = L.ekpnp_mlp_create(ctypes.byref(self.cfg), ctypes.byref(h))
+        if rc != 0:
+            raise EkpnpError(rc, "ekpnp_mlp_create failed")
+        self._h = h
+        self.n_params = int(L.ekpnp_mlp_num_params(h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib().ekpnp_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise EkpnpError(rc, _lib().ekpnp_mlp_last_error(self._h).decode())
+
+    def layer_shapes(self):
+        """[(out, in_padded), ...] per layer, in parameter-vector order."""
+        c = self.cfg
+        dims = [((c.n_in + 31) // 32) * 32] + [c.n_hidden] * c.n_layers + [c.n_out]
+        return [(dims[i + 1], dims[i]) for i in range(len(dims) - 1)]
+
+    def get_params(self) -> np.ndarray:
+        out = np.empty(self.n_params, dtype=np.float32)
+        self._check(_lib().ekpnp_mlp_get_params(self._h, _fp(out)))
+        return out
+
+    def set_params(self, p) -> None:
+        arr = _f32(p, (self.n_params,))
+        self._check(_lib().ekpnp_mlp_set_params(self._h, _fp(arr)))
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x)
+        B = x.shape[0]
+        y = np.empty((B, self.cfg.n_out), dtype=np.float32)
+        for s in range(0, B, self.cfg.max_batch):
+            xb = np.ascontiguousarray(x[s:s + self.cfg.max_batch])
+            yb = np.empty((xb.shape[0], self.cfg.n_out), dtype=np.float32)
+            self._check(_lib().ekpnp_mlp_forward(self._h, _fp(xb), xb.shape[0], _fp(yb)))
+            y[s:s + xb.shape[0]] = yb
+        return y
+
+    def loss_grad(self, x, t):
+        x, t = _f32(x), _f32(t)
+        loss = ctypes.c_float()
+        g = np.empty(self.n_params, dtype=np.float32)
+        self._check(_lib().ekpnp_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], ctypes.byref(loss), _fp(g)))
+        return float(loss.value), g
+
+    def train_step(self, x, t, lr: float) -> float:
+        x, t = _f32(x), _f32(t)
+        loss = ctypes.c_float()
+        self._check(_lib().ekpnp_mlp_train_step(self._h, _fp(x), _fp(t), x.shape[0], lr, ctypes.byref(loss)))
+        return float(loss.value)
+
+    def set_adam(self, beta1=0.9, beta2=0.999, eps=1e-8, weight_decay=0.0):
+        self._check(_lib().ekpnp_mlp_set_adam(self._h, beta1, beta2, eps, weight_decay))
+
+
+# ---------------------------------------------------------------------------
+# features, targets, resampling
+# ---------------------------------------------------------------------------
+def features(zeta, kappa_h, e_x, ny) -> np.ndarray:
+    """[n, 4] network inputs, each roughly in [-1, 1]."""
+    zeta, kappa_h, e_x, ny = (np.atleast_1d(np.asarray(v, dtype=np.float64)) for v in (zeta, kappa_h, e_x, ny))
+    return np.stack([zeta / 2.0 + 1.0,
+                     np.log(kappa_h) / np.log(50.0) * 2.0 - 1.0,
+                     (np.log10(e_x) + 4.0) / 1.0,
+                     ny / 512.0 - 1.0], axis=1).astype(np.float32)
+
+
+def resample(profile: np.ndarray, n: int) -> np.ndarray:
+    """Linear resampling of a profile on nodes 0..len-1 to n points on the same span."""
+    src = np.linspace(0.0, 1.0, profile.shape[-1])
+    dst = np.linspace(0.0, 1.0, n)
+    return np.interp(dst, src, profile)
+
+
+def hs_scale(p: ChannelParams) -> float:
+    """u_x scale of a channel: E_x (-zeta) force_coef / (poisson_coef rho0 nu)."""
+    return p.e_applied * (-p.psi_lo) * (p.force_coef / p.poisson_coef) / (p.rho0 * p.nu)
+
+
+def targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:
+    """Normalised 1024-vector from x-averaged wall-normal profiles (length ny)."""
+    z = p.psi_lo
+    out = np.concatenate([resample(psi / z, N_PTS),
+                          resample(np.log(np.maximum(cpos, 1e-300)) / (-z), N_PTS),
+                          resample(np.log(np.maximum(cneg, 1e-300)) / z, N_PTS),
+                          resample(ux / hs_scale(p), N_PTS)])
+    return out.astype(np.float32)
+
+
+def profiles(p: ChannelParams, y_norm: np.ndarray):
+    """Inverse of targets(): (psi, c+, c-, u_x) on the channel's ny rows."""
+    z = p.psi_lo
+    ny = p.ny
+    parts = [resample(y_norm[i * N_PTS:(i + 1) * N_PTS].astype(np.float64), ny) for i in range(N_FIELDS)]
+    psi = parts[0] * z
+    cpos = np.exp(parts[1] * (-z))
+    cneg = np.exp(parts[2] * z)
+    ux = parts[3] * hs_scale(p)
+    # walls exactly as the boundary conditions hold them
+    psi[0], psi[-1] = p.psi_lo, p.psi_hi
+    cpos[0], cpos[-1] = p.cpos_lo, p.cpos_hi
+    cneg[0], cneg[-1] = p.cneg_lo, p.cneg_hi
+    return psi, cpos, cneg, ux
+
+
+def channel_of(zeta, kappa_h, e_x, nx, ny, tau=0.8, diff=0.05) -> ChannelParams:
+    from . import lattice_params
+    return lattice_params(int(nx), int(ny), tau=tau, zeta=float(zeta), kappa_h=float(kappa_h), e_x=float(e_x),
+                          diff=diff)
+
+
+def stable(p: ChannelParams, max_kappa_dy=0.4, max_drift=0.05) -> bool:
+    """Acceptance rule for sampled channels: the Debye length is resolved
+    (kappa dy <= max_kappa_dy) and the Gouy-Chapman wall drift of the ions,
+    D * 2 kappa sinh(|zeta|/2), stays far below the lattice sound speed."""
+    kappa = np.sqrt(2.0 * p.poisson_coef)
+    drift = p.diff_pos * 2.0 * kappa * np.sinh(abs(p.psi_lo) / 2.0)
+    return bool(kappa * p.dy <= max_kappa_dy and drift <= max_drift)
+
+
+def sample_channels(n: int, seed: int, nx_choices=(32, 64, 128, 256), ny_choices=None,
+                    zeta_range=(-4.0, -0.4), kappa_h_range=(1.0, 50.0), e_x_range=(1e-5, 1e-3)):
+    """Config-2-style channels drawn from a seeded stream; draws that fail
+    stable() are redrawn.  Returns (list of ChannelParams, number rejected)."""
+    if ny_choices is None:
+        ny_choices = tuple(range(64, 1025, 64))
+    rng = np.random.default_rng(seed)
+    out, rejected = [], 0
+    while len(out) < n:
+        nx = int(rng.choice(nx_choices))
+        ny = int(rng.choice(ny_choices))
+        zeta = float(rng.uniform(*zeta_range))
+        kh = float(np.exp(rng.uniform(np.log(kappa_h_range[0]), np.log(kappa_h_range[1]))))
+        ex = float(np.exp(rng.uniform(np.log(e_x_range[0]), np.log(e_x_range[1]))))
+        p = channel_of(zeta, kh, ex, nx, ny)
+        if stable(p):
+            out.append(p)
+        else:
+            rejected += 1
+    return out, rejected
+
+
+def channel_features(ps: Sequence[ChannelParams]) -> np.ndarray:
+    kh = [np.sqrt(2.0 * p.poisson_coef) * 0.5 * (p.ny - 1) for p in ps]
+    return features([p.psi_lo for p in ps], kh, [p.e_applied for p in ps], [p.ny for p in ps])

@@ -80,8 +80,12 @@ enum {
   EKPNP_INIT_COLD_NOISE = 0, /* rho0, u = 0, c+- = c_bulk(1 + noise*U(-1,1))
                                 inside, wall values on the walls, then one
                                 Poisson solve for psi and E                 */
-  EKPNP_INIT_REFERENCE = 1   /* gpu_initialization values (LBM.cu:147-160):
+  EKPNP_INIT_REFERENCE = 1,  /* gpu_initialization values (LBM.cu:147-160):
                                 rho0, c+- = 0, psi = psi_lo, u = E = 0      */
+  EKPNP_INIT_PB_PICARD = 2   /* the reference's start: gpu_initialization, then
+                                pb_iterations x { c+- = c_bulk exp(-+inv_vt psi);
+                                Poisson; psi = w psi + (1-w) psi_old }
+                                (LBM.cu:95-145), entirely on the device     */
 };
 
 /* indices into the per-channel record of ekpnp_reductions */
@@ -133,6 +137,10 @@ typedef struct ekpnp_channel_params {
   double uw;           /* top-wall velocity, LBM.h:46 */
   double lx;           /* Lx, the x period used by kx (main.cu:120-128)     */
   double init_noise;   /* relative amplitude of the cold-start noise        */
+  double inv_thermal_voltage; /* e/(kB T) in units of 1/psi: the Boltzmann
+                          factor exp(-+ inv_vt psi) of the PB start (LBM.cu:143-144) */
+  double pb_omega;     /* PB_omega, Picard under-relaxation (LBM.h:91)      */
+  double pb_iterations; /* Picard iterations of the PB start (LBM.cu:116: 501) */
 } ekpnp_channel_params;
 
 typedef struct ekpnp_options {

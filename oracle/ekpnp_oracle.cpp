@@ -648,13 +648,28 @@ int ekpnp_oracle_initialization(ekpnp_oracle* o, int mode, uint64_t seed, int ch
   std::fill(o->ux.begin(), o->ux.end(), 0.0);
   std::fill(o->uy.begin(), o->uy.end(), 0.0);
   std::fill(o->rho.begin(), o->rho.end(), p.rho0);
-  if (mode == EKPNP_INIT_REFERENCE) {
+  if (mode == EKPNP_INIT_REFERENCE || mode == EKPNP_INIT_PB_PICARD) {
     // LBM.cu:147-160
     std::fill(o->cp.begin(), o->cp.end(), 0.0);
     std::fill(o->cn.begin(), o->cn.end(), 0.0);
     std::fill(o->psi.begin(), o->psi.end(), p.psi_lo);
     std::fill(o->ex.begin(), o->ex.end(), 0.0);
     std::fill(o->ey.begin(), o->ey.end(), 0.0);
+    if (mode == EKPNP_INIT_REFERENCE) return 0;
+    // LBM.cu:107-127: 501 Picard iterations with under-relaxation PB_omega
+    std::vector<double> psi_old = o->psi;
+    const int iters = (int)p.pb_iterations;
+    for (int it = 0; it < iters; ++it) {
+      for (size_t i = 0; i < o->n; ++i) {  // gpu_PBE, LBM.cu:139-145
+        o->cp[i] = p.c_bulk * std::exp(-p.inv_thermal_voltage * o->psi[i]);
+        o->cn[i] = p.c_bulk * std::exp(p.inv_thermal_voltage * o->psi[i]);
+      }
+      fast_poisson(*o, poisson_method);
+      for (size_t i = 0; i < o->n; ++i) {  // gpu_PBE_phi, LBM.cu:132-137
+        o->psi[i] = p.pb_omega * o->psi[i] + (1.0 - p.pb_omega) * psi_old[i];
+        psi_old[i] = o->psi[i];
+      }
+    }
     return 0;
   }
   if (mode != EKPNP_INIT_COLD_NOISE) return -1;

@@ -10,13 +10,16 @@ import ekpnp
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "ekpnp.h")
+HEADERS = [HEADER, os.path.join(ROOT, "include", "ekpnp_mlp.h")]
 
 
 def declared_functions():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"\b(ekpnp_[a-z_0-9]+)\s*\(", text)
-    return sorted(set(names))
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(ekpnp_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_entry_point():
@@ -26,7 +29,8 @@ def test_library_exports_every_declared_entry_point():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     # the Python mirror binds each of them with a signature
-    bound = {n for n, _, _ in ekpnp.SIGNATURES}
+    from ekpnp import surrogate
+    bound = {n for n, _, _ in ekpnp.SIGNATURES + surrogate.MLP_SIGNATURES}
     assert set(names) == bound, set(names) ^ bound
 
 

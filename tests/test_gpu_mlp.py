@@ -1,0 +1,81 @@
+"""Surrogate MLP kernels (BF16 MFMA, FP32 accumulation) against the FP64
+reference with BF16 rounding emulated at the same points."""
+import numpy as np
+import pytest
+
+import mlp_ref as R
+from ekpnp import surrogate as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _net(n_hidden=64, n_layers=2, n_out=64, seed=5, max_batch=96):
+    m = S.MLP(n_in=4, n_hidden=n_hidden, n_layers=n_layers, n_out=n_out, max_batch=max_batch, seed=seed)
+    shp = R.shapes(4, n_hidden, n_layers, n_out)
+    assert m.layer_shapes() == shp
+    return m, shp
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, float) - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("batch", [1, 37, 96])
+def test_forward_matches_reference(batch):
+    m, shp = _net()
+    p = m.get_params().astype(np.float64)
+    x = np.random.default_rng(batch).uniform(-1, 1, (batch, 4))
+    y = m.forward(x)
+    yr, _ = R.forward(p, x, shp, emulate=True)
+    e = _rel(y, yr)
+    print("forward rel", batch, e, flush=True)
+    assert e < 5e-3
+
+
+def test_gradients_match_reference():
+    m, shp = _net(n_hidden=96, n_layers=3, n_out=128)
+    p = m.get_params().astype(np.float64)
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, (80, 4))
+    t = rng.standard_normal((80, 128)) * 0.5
+    loss, g = m.loss_grad(x, t)
+    lr, gr = R.loss_grad(p, x, t, shp, emulate=True)
+    assert abs(loss - lr) <= 1e-3 * lr
+    off = 0
+    for li, (o, i) in enumerate(shp):
+        for name, n in (("W", o * i), ("b", o)):
+            e = _rel(g[off:off + n], gr[off:off + n])
+            print(f"grad layer {li} {name} rel {e:.2e}", flush=True)
+            assert e < 2e-2, (li, name, e)
+            off += n
+    # the emulated reference stays close to the pure FP64 gradient
+    _, g64 = R.loss_grad(p, x, t, shp, emulate=False)
+    assert _rel(g, g64) < 5e-2
+
+
+def test_full_size_network_forward():
+    # the surrogate's own shape: 4 -> 4 x 256 GELU -> 1024
+    m = S.MLP(max_batch=256, seed=11)
+    shp = R.shapes(4, 256, 4, 1024)
+    p = m.get_params().astype(np.float64)
+    x = np.random.default_rng(3).uniform(-1, 1, (200, 4))
+    e = _rel(m.forward(x), R.forward(p, x, shp, emulate=True)[0])
+    print("full forward rel", e, flush=True)
+    assert e < 5e-3
+
+
+def test_training_reduces_loss_on_smooth_targets():
+    m, shp = _net(n_hidden=64, n_layers=2, n_out=64, max_batch=128)
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (128, 4))
+    yy = np.linspace(0, 1, 64)
+    t = np.tanh(x[:, :1] * 2 + yy[None, :] * x[:, 1:2]) + 0.3 * x[:, 2:3]
+    first = m.train_step(x, t, 3e-3)
+    for _ in range(300):
+        last = m.train_step(x, t, 3e-3)
+    print("loss", first, "->", last, flush=True)
+    assert last < 0.05 * first
+    # params round-trip and Adam state reset
+    p = m.get_params()
+    m.set_params(p)
+    assert np.array_equal(m.get_params(), p)

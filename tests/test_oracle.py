@@ -369,3 +369,19 @@ def test_coupled_steady_state_converges_to_pb_and_hs():
     assert boltz[1] < 0.6 * boltz[0]
     assert 0.4 < ratio < 0.6, (errs, ratio)
     assert hs[1] < 0.03 and hs[1] < 0.7 * hs[0], hs
+
+
+def test_pb_picard_start_reaches_discrete_pb_solution():
+    # the reference's start (LBM.cu:106-127): under-relaxed Picard on the
+    # Boltzmann concentrations converges to the discrete Poisson-Boltzmann
+    # solution on the same nodes (Newton restatement above)
+    ny = 34
+    p = lattice(4, ny, zeta=-1.0, kappa_h=3.0)
+    o = O.Oracle(p)
+    o.initialization(ekpnp.PB_PICARD, 0, 0)
+    psi = o.get_field("psi")[:, 0]
+    np.testing.assert_allclose(psi, _pb_newton(ny, -1.0, p.poisson_coef), atol=1e-10)
+    # concentrations are the Boltzmann fields of the iterate before the last solve
+    np.testing.assert_allclose(o.get_field("cpos")[:, 0], np.exp(-psi), rtol=1e-9)
+    np.testing.assert_allclose(o.get_field("cneg")[:, 0], np.exp(psi), rtol=1e-9)
+    np.testing.assert_array_equal(o.get_field("ux"), 0.0)
