@@ -43,8 +43,63 @@ def workload(name):
         return [p], "fp64", 0, "single channel 64x128, tau 0.8, zeta -1, kappa_h 8, E_x 1e-4, FP64"
     if name == "c4":
         p = ekpnp.lattice_params(8192, 2048, tau=0.8, zeta=-1.5, kappa_h=64.0, e_x=1e-4)
-        return [p], "fp32", 4, "single channel 8192x2048, kappa_h 64, E_x 1e-4, uniform zeta -1.5"
+        return [p], "fp32", 4, ("single channel 8192x2048, wall zeta(x) = -1.5 + sum_k a_k sin(2 pi k x/nx "
+                                "+ phi_k), k = 1..4, seed 4, kappa_h 64, E_x 1e-4")
     raise SystemExit(f"unknown config {name}")
+
+
+def wall_zeta(nx, seed, zeta0=-1.5, modes=4, amp=0.5):
+    """zeta(x) = zeta0 + sum_k a_k sin(2 pi k x / nx + phi_k), a_k ~ U[0, amp],
+    phi_k ~ U[0, 2 pi), drawn from a seeded numpy stream."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.0, amp, modes)
+    phi = rng.uniform(0.0, 2 * np.pi, modes)
+    x = np.arange(nx)
+    return zeta0 + sum(a[k] * np.sin(2 * np.pi * (k + 1) * x / nx + phi[k]) for k in range(modes))
+
+
+def apply_walls(s, name, params, seed):
+    """Per-x wall potential and Boltzmann wall concentrations (config 4)."""
+    import numpy as np
+
+    if name != "c4":
+        return
+    for ch, p in enumerate(params):
+        z = wall_zeta(p.nx, seed)
+        s.set_wall_profile(ch, psi_lo=z, psi_hi=z, cpos_lo=np.exp(-z), cpos_hi=np.exp(-z),
+                           cneg_lo=np.exp(z), cneg_hi=np.exp(z))
+
+
+def calibrate_copy():
+    """A plain device copy of known size: 1 GiB read + 1 GiB written per
+    dispatch, three dispatches."""
+    import torch
+
+    a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(3):
+        b.copy_(a)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / 3
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "ms_per_copy": round(ms, 4),
+                      "GBps": round(2 * (1 << 30) / (ms * 1e-3) / 1e9, 1)}), flush=True)
+
+
+def measured_traffic(config, precision):
+    """HBM bytes per k_lbm launch from the committed counter summary of this
+    workload (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or
+    None when no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", f"k_lbm_{config}_{precision}_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
 
 
 def cpu_baseline(params, seconds_target=15.0):
@@ -80,11 +135,17 @@ def main():
     ap.add_argument("--stage-steps", type=int, default=20)
     ap.add_argument("--compute", choices=("fp32", "fp64"), default="fp64",
                     help="collision arithmetic for FP32 population storage")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB device copy three times (the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU: channels and y-slabs would shard "
+                                   "across GPUs, but no multi-GPU path is built, so --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
 
     import ekpnp
 
@@ -93,6 +154,7 @@ def main():
     print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     s = ekpnp.Solver(params, precision=precision, compute_fp64=(args.compute == "fp64"))
+    apply_walls(s, args.config, params, seed)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.step(args.warmup)
@@ -135,7 +197,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.config, precision),
         },
         "finite": bool(red[:, 6].sum() == 0),
     }

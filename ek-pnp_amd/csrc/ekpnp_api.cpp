@@ -49,13 +49,14 @@ struct ekpnp_ctx {
   int elem = 4;  // bytes per population value
   std::vector<ChanDesc> desc;
   std::vector<ChannelTiles> ctiles;
-  long long total_nodes = 0, total_spec = 0, total_pops = 0;
+  long long total_nodes = 0, total_spec = 0, total_pops = 0, total_refs = 0;
   int total_nx = 0;
   // host copies
   std::vector<int2> lbm_tiles;
   std::vector<int4> fft_tiles;
   std::vector<int2> tri_tiles;
   std::vector<int> first_tile;
+  std::vector<int2> row_tiles;
   std::vector<double> walls;  // 6 planes x total_nx
   // device
   void* lat[2] = {nullptr, nullptr};
@@ -71,6 +72,9 @@ struct ekpnp_ctx {
   double2* d_tw = nullptr;
   double2* d_spec = nullptr;
   double* d_ux_prev = nullptr;
+  double* d_refs = nullptr;      // storage references in use, [3][ny] per channel
+  double* d_refs_new = nullptr;  // staging for a change of reference
+  int2* d_row_tiles = nullptr;
   double* d_partials = nullptr;
   double* d_red = nullptr;
   size_t dev_bytes = 0;
@@ -83,6 +87,7 @@ struct ekpnp_ctx {
   bool has_fields = false, has_pops = false;
   long long steps = 0;
   long long snapshot_step = -1;
+  long long rebase_step = 0;  // step count at the last change of storage reference
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
   std::string err;
 };
@@ -127,7 +132,7 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.nx = p.nx;
   d.ny = p.ny;
   d.nk = p.nx / 2 + 1;
-  d.nk_stride = (d.nk + 7) / 8 * 8;  // spectrum rows start on 128-byte lines
+  d.nk_stride = (d.nk + 7) / 8 * 8;  // modes rounded up to whole blocks of 8
   d.active = 1;
   d.swap = 0;
   d.flags = (int)p.flags;
@@ -152,11 +157,6 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.src_m = 1.0 - 0.5 * dt * omega_minus;
   d.dt = dt;
   d.half_dt = 0.5 * dt;
-  if (precision == EKPNP_POP_FP32) {
-    d.rho_ref = p.rho0;
-    d.cp_ref = p.c_bulk;
-    d.cn_ref = p.c_bulk;
-  }
   d.kp = p.mob_pos;
   d.kn = p.mob_neg;
   d.fcoef = p.force_coef;
@@ -175,7 +175,7 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   // tridiagonal partition: C chunks of L rows, G = TRI_WG / C modes per WG
   const int M = p.ny - 2;
   int C = pow2_ceil((M + 15) / 16);
-  C = std::max(8, std::min(64, C));
+  C = std::max(8, std::min(128, C));
   d.tri_C = C;
   d.tri_L = (M + C - 1) / C;
   d.tri_G = TRI_WG / C;
@@ -204,6 +204,9 @@ StepBuffers make_buffers(ekpnp_ctx* c) {
   b.red_out = c->d_red;
   b.red_first_tile = c->d_first_tile;
   b.n_channels = c->nchan;
+  b.refs = c->d_refs;
+  b.row_tiles = c->d_row_tiles;
+  b.n_row_tiles = (int)c->row_tiles.size();
   return b;
 }
 
@@ -220,7 +223,8 @@ void release(ekpnp_ctx* c) {
   c->graphs.clear();
   void* ptrs[] = {c->lat[0], c->lat[1], c->d_desc, c->d_lbm_tiles, c->d_fft_tiles,
                   c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
-                  c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red};
+                  c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red,
+                  c->d_refs, c->d_refs_new, c->d_row_tiles};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -445,7 +449,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   c->ctiles.resize(n_channels);
 
   // layout
-  long long pop = 0, fld = 0, spec = 0;
+  long long pop = 0, fld = 0, spec = 0, refs = 0;
   int wall = 0;
   // every per-channel block starts on a 64-element boundary
   auto pad64 = [](long long v) { return (v + 63) / 64 * 64; };
@@ -456,18 +460,23 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     d.pop_off = pop;
     d.fld_off = fld;
     d.spec_off = spec;
+    d.ref_off = refs;
     d.wall_off = wall;
     d.tw_off = wall;
     pop += 27 * d.plane;
     fld += pad64(nn);
     spec += pad64((long long)d.ny * d.nk_stride);
+    refs += pad64(3LL * d.ny);
     wall += (int)pad64(d.nx);
     c->desc[i] = d;
   }
   c->total_pops = pop;
   c->total_nodes = fld;
   c->total_spec = spec;
+  c->total_refs = refs;
   c->total_nx = wall;
+  for (int i = 0; i < n_channels; ++i)
+    for (int y = 0; y < c->desc[i].ny; ++y) c->row_tiles.push_back(make_int2(i, y));
 
   // LBM tiles, channel-major
   c->first_tile.resize(n_channels + 1);
@@ -550,6 +559,9 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = dalloc(c, &c->d_tw, tw.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_spec, c->total_spec)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_ux_prev, c->total_nodes)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_refs, c->total_refs)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_refs_new, c->total_refs)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_row_tiles, c->row_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_partials, 8 * c->lbm_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_red, 8 * (size_t)n_channels)) != hipSuccess) return e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return e;
@@ -571,6 +583,9 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = hipMemcpy(c->d_first_tile, c->first_tile.data(), c->first_tile.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_walls, c->walls.data(), c->walls.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_row_tiles, c->row_tiles.data(), c->row_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_refs, 0, c->total_refs * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_refs_new, 0, c->total_refs * sizeof(double))) != hipSuccess) return e;
     const size_t npop = (size_t)c->total_pops;
     if ((e = hipMemset(c->lat[0], 0, npop * c->elem)) != hipSuccess) return e;
     if ((e = hipMemset(c->lat[1], 0, npop * c->elem)) != hipSuccess) return e;
@@ -728,10 +743,14 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
     d.active = 1;
   }
   if (reset) HIP_OR_FAIL(ctx, sync_desc(ctx));
+  // FP32 storage: populations are stored as deviations from w_i times the
+  // row means of the fields they start from
+  if (ctx->opt.precision == EKPNP_POP_FP32) HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs, ctx->stream));
   HIP_OR_FAIL(ctx, launch_init_equilibrium(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   ctx->state_q = true;
   ctx->has_pops = true;
+  ctx->rebase_step = ctx->steps;
   return EKPNP_OK;
 }
 
@@ -749,6 +768,22 @@ int ekpnp_fast_poisson(ekpnp_ctx* ctx) {
   if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "no fields");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, enqueue_poisson(ctx, -1));
+  return EKPNP_OK;
+}
+
+int ekpnp_rebase(ekpnp_ctx* ctx) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
+  ctx->rebase_step = ctx->steps;
+  if (ctx->opt.precision != EKPNP_POP_FP32) return EKPNP_OK;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  // the macroscopic planes hold the moments of the last step (or the fields
+  // the populations were initialised from)
+  HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs_new, ctx->stream));
+  HIP_OR_FAIL(ctx, launch_rebase(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1],
+                                 ctx->d_refs_new, ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_refs, ctx->d_refs_new, ctx->total_refs * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
   return EKPNP_OK;
 }
 
@@ -776,6 +811,10 @@ int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
     }
   }
   ctx->steps += nsteps;
+  // FP32 storage follows the drifting row means so that the stored
+  // deviations stay small
+  if (ctx->opt.precision == EKPNP_POP_FP32 && ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL)
+    return ekpnp_rebase(ctx);
   return EKPNP_OK;
 }
 
@@ -841,7 +880,9 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   const ChanDesc& d = ctx->desc[channel];
   const size_t nn = (size_t)d.nx * d.ny;
-  const double ref = species == 0 ? d.rho_ref : (species == 1 ? d.cp_ref : d.cn_ref);
+  std::vector<double> ref(d.ny);
+  HIP_OR_FAIL(ctx, hipMemcpy(ref.data(), ctx->d_refs + d.ref_off + (size_t)species * d.ny, d.ny * sizeof(double),
+                             hipMemcpyDeviceToHost));
   const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
   const char* base = (const char*)src + off * ctx->elem;
   if (ctx->elem == 8) {
@@ -852,7 +893,7 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
     for (size_t i = 0; i < 9 * nn; ++i) host_out[i] = (double)tmp[i];
   }
   for (int dd = 0; dd < 9; ++dd)
-    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += W[dd] * ref;
+    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += W[dd] * ref[i / d.nx];
   return EKPNP_OK;
 }
 
@@ -869,19 +910,21 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double
   ctx->state_q = true;
   const ChanDesc& d = ctx->desc[channel];
   const size_t nn = (size_t)d.nx * d.ny;
-  const double ref = species == 0 ? d.rho_ref : (species == 1 ? d.cp_ref : d.cn_ref);
   const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
   char* base = (char*)ctx->lat[chan_lattice(ctx, channel)] + off * ctx->elem;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  std::vector<double> ref(d.ny);
+  HIP_OR_FAIL(ctx, hipMemcpy(ref.data(), ctx->d_refs + d.ref_off + (size_t)species * d.ny, d.ny * sizeof(double),
+                             hipMemcpyDeviceToHost));
   if (ctx->elem == 8) {
     std::vector<double> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - W[dd] * ref;
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - W[dd] * ref[i / d.nx];
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 8, tmp.data(), nn * 8, nn * 8, 9, hipMemcpyHostToDevice));
   } else {
     std::vector<float> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - W[dd] * ref);
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - W[dd] * ref[i / d.nx]);
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 4, tmp.data(), nn * 4, nn * 4, 9, hipMemcpyHostToDevice));
   }
   ctx->has_pops = true;
@@ -973,6 +1016,141 @@ int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_s
     }
   }
   return thaw();
+}
+
+int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (!out) return fail(ctx, EKPNP_ERR_INVALID, "null output");
+  *out = ctx->params[channel];
+  return EKPNP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Binary checkpoint.  Layout (native byte order):
+//   char[8] "EKPNPCK1"; int32 nchan, precision, state_q, has_pops;
+//   int64 steps, snapshot_step, rebase_step;
+//   per channel: int32 nx, ny;
+//   per channel: populations [27][ny][nx] in the storage type, references
+//   [3][ny], psi, rho, ux, uy, c+, c- [ny][nx], wall planes [6][nx],
+//   residual snapshot [ny][nx] (FP64 unless stated).
+// ---------------------------------------------------------------------------
+namespace {
+const char kCkptMagic[8] = {'E', 'K', 'P', 'N', 'P', 'C', 'K', '1'};
+
+struct FileCloser {
+  FILE* f;
+  ~FileCloser() {
+    if (f) std::fclose(f);
+  }
+};
+}  // namespace
+
+int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
+  if (!ctx || !path) return EKPNP_ERR_INVALID;
+  if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "nothing to save: no fields");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) return fail(ctx, EKPNP_ERR_IO, std::string("cannot open ") + path);
+  FileCloser closer{fp};
+  bool ok = true;
+  auto put = [&](const void* p, size_t n) { ok = ok && std::fwrite(p, 1, n, fp) == n; };
+  const int32_t hdr[4] = {ctx->nchan, ctx->opt.precision, ctx->state_q ? 1 : 0, ctx->has_pops ? 1 : 0};
+  const int64_t cnt[3] = {ctx->steps, ctx->snapshot_step, ctx->rebase_step};
+  put(kCkptMagic, 8);
+  put(hdr, sizeof hdr);
+  put(cnt, sizeof cnt);
+  for (const auto& d : ctx->desc) {
+    const int32_t sh[2] = {d.nx, d.ny};
+    put(sh, sizeof sh);
+  }
+  std::vector<char> buf;
+  for (int ch = 0; ch < ctx->nchan && ok; ++ch) {
+    const ChanDesc& d = ctx->desc[ch];
+    const size_t nn = (size_t)d.nx * d.ny, e = ctx->elem;
+    buf.resize(std::max<size_t>(27 * nn * e, (M_COUNT + 1) * nn * sizeof(double)));
+    const char* lat = (const char*)ctx->lat[chan_lattice(ctx, ch)] + (size_t)d.pop_off * e;
+    HIP_OR_FAIL(ctx, hipMemcpy2D(buf.data(), nn * e, lat, (size_t)d.plane * e, nn * e, 27, hipMemcpyDeviceToHost));
+    put(buf.data(), 27 * nn * e);
+    std::vector<double> refs(3 * (size_t)d.ny);
+    HIP_OR_FAIL(ctx, hipMemcpy(refs.data(), ctx->d_refs + d.ref_off, refs.size() * sizeof(double),
+                               hipMemcpyDeviceToHost));
+    put(refs.data(), refs.size() * sizeof(double));
+    double* fb = reinterpret_cast<double*>(buf.data());
+    HIP_OR_FAIL(ctx, hipMemcpy(fb, ctx->d_psi + d.fld_off, nn * sizeof(double), hipMemcpyDeviceToHost));
+    for (int k = 0; k < M_COUNT; ++k)
+      HIP_OR_FAIL(ctx, hipMemcpy(fb + (k + 1) * nn, ctx->d_macro + k * ctx->total_nodes + d.fld_off,
+                                 nn * sizeof(double), hipMemcpyDeviceToHost));
+    put(fb, (M_COUNT + 1) * nn * sizeof(double));
+    for (int a = 0; a < 6; ++a) put(&ctx->walls[a * (size_t)ctx->total_nx + d.wall_off], d.nx * sizeof(double));
+    HIP_OR_FAIL(ctx, hipMemcpy(fb, ctx->d_ux_prev + d.fld_off, nn * sizeof(double), hipMemcpyDeviceToHost));
+    put(fb, nn * sizeof(double));
+  }
+  if (!ok) return fail(ctx, EKPNP_ERR_IO, std::string("short write to ") + path);
+  return EKPNP_OK;
+}
+
+int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
+  if (!ctx || !path) return EKPNP_ERR_INVALID;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return fail(ctx, EKPNP_ERR_IO, std::string("cannot open ") + path);
+  FileCloser closer{fp};
+  auto get = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
+  char magic[8];
+  int32_t hdr[4];
+  int64_t cnt[3];
+  if (!get(magic, 8) || std::memcmp(magic, kCkptMagic, 8) != 0 || !get(hdr, sizeof hdr) || !get(cnt, sizeof cnt))
+    return fail(ctx, EKPNP_ERR_IO, "not a checkpoint file");
+  if (hdr[0] != ctx->nchan || hdr[1] != ctx->opt.precision)
+    return fail(ctx, EKPNP_ERR_INVALID, "checkpoint channel count or precision differs from the context");
+  for (const auto& d : ctx->desc) {
+    int32_t sh[2];
+    if (!get(sh, sizeof sh)) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    if (sh[0] != d.nx || sh[1] != d.ny) return fail(ctx, EKPNP_ERR_INVALID, "checkpoint channel shape differs");
+  }
+  // every channel's state goes to lattice cur
+  for (auto& d : ctx->desc) {
+    d.swap = 0;
+    d.active = 1;
+  }
+  HIP_OR_FAIL(ctx, sync_desc(ctx));
+  std::vector<char> buf;
+  for (int ch = 0; ch < ctx->nchan; ++ch) {
+    const ChanDesc& d = ctx->desc[ch];
+    const size_t nn = (size_t)d.nx * d.ny, e = ctx->elem;
+    buf.resize(std::max<size_t>(27 * nn * e, (M_COUNT + 1) * nn * sizeof(double)));
+    if (!get(buf.data(), 27 * nn * e)) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    char* lat = (char*)ctx->lat[ctx->cur] + (size_t)d.pop_off * e;
+    HIP_OR_FAIL(ctx, hipMemcpy2D(lat, (size_t)d.plane * e, buf.data(), nn * e, nn * e, 27, hipMemcpyHostToDevice));
+    std::vector<double> refs(3 * (size_t)d.ny);
+    if (!get(refs.data(), refs.size() * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_refs + d.ref_off, refs.data(), refs.size() * sizeof(double),
+                               hipMemcpyHostToDevice));
+    double* fb = reinterpret_cast<double*>(buf.data());
+    if (!get(fb, (M_COUNT + 1) * nn * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_psi + d.fld_off, fb, nn * sizeof(double), hipMemcpyHostToDevice));
+    for (int k = 0; k < M_COUNT; ++k)
+      HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_macro + k * ctx->total_nodes + d.fld_off, fb + (k + 1) * nn,
+                                 nn * sizeof(double), hipMemcpyHostToDevice));
+    for (int a = 0; a < 6; ++a) {
+      double* w = &ctx->walls[a * (size_t)ctx->total_nx + d.wall_off];
+      if (!get(w, d.nx * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+      HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_walls + a * (size_t)ctx->total_nx + d.wall_off, w, d.nx * sizeof(double),
+                                 hipMemcpyHostToDevice));
+    }
+    if (!get(fb, nn * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_ux_prev + d.fld_off, fb, nn * sizeof(double), hipMemcpyHostToDevice));
+  }
+  ctx->state_q = hdr[2] != 0;
+  ctx->has_pops = hdr[3] != 0;
+  ctx->has_fields = true;
+  ctx->steps = cnt[0];
+  ctx->snapshot_step = cnt[1];
+  ctx->rebase_step = cnt[2];
+  return EKPNP_OK;
 }
 
 int ekpnp_num_channels(const ekpnp_ctx* ctx) { return ctx ? ctx->nchan : 0; }

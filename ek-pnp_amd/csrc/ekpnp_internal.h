@@ -21,18 +21,19 @@ constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 8
 constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
-constexpr int MAX_INTERIOR_ROWS = 2048;  // 64 chunks x 32 rows per tridiagonal column
+constexpr int MAX_INTERIOR_ROWS = 2048;  // 128 chunks x 16 rows per tridiagonal column
 
 struct ChanDesc {
   int nx, ny, nk, flags;        // nk = nx/2 + 1 spectral columns
-  int nk_stride;                // spectrum row stride (nk rounded up to 8)
+  int nk_stride;                // nk rounded up to a multiple of 8 (spectrum blocks)
   int active;                   // 0: the step kernels skip this channel (frozen)
   int swap;                     // 1: this channel's state is in the step's dst lattice
   int pad0;
   long long pop_off;            // first element of this channel's 27 planes
   long long plane;              // population plane stride (nx*ny rounded up to 64)
   long long fld_off;            // first element of this channel's [ny][nx] fields
-  long long spec_off;           // first double2 of this channel's [ny][nk] spectrum
+  long long spec_off;           // first double2 of this channel's spectrum blocks
+  long long ref_off;            // first double of this channel's [3][ny] storage references
   int wall_off;                 // offset into each wall plane
   int tw_off;                   // offset of this channel's twiddles e^{-2 pi i j/nx}
   int pow2;                     // nx is a power of two (Stockham path)
@@ -45,7 +46,6 @@ struct ChanDesc {
   double om_pn, om_mn;          // negative ion
   double src_p, src_m;          // 1 - omega+- dt/2 (Guo source split)
   double dt, half_dt;
-  double rho_ref, cp_ref, cn_ref;  // storage offsets (0 for FP64 storage)
   double kp, kn;                // mobilities
   double fcoef, e_app, exf;     // body force
   double drift_app;             // e_app if ions drift in the applied field else 0
@@ -76,12 +76,15 @@ struct StepBuffers {
   double* walls;                // 6 wall planes
   int walls_stride;             // = total nx
   const double2* tw;            // twiddles
-  double2* spec;                // spectrum [ny][nk_stride] per channel
+  double2* spec;                // spectrum [nk_stride/8][ny][8] per channel
   double* ux_prev;              // residual snapshot
   double* partials;             // reduction partials [n_lbm_tiles][8]
   double* red_out;              // reductions [n_channels][8]
   const int* red_first_tile;    // first LBM tile of each channel (n_channels + 1)
   int n_channels;
+  double* refs;                 // storage references, [3][ny] per channel (rho, c+, c-)
+  const int2* row_tiles;        // (channel, row) per row-means work-group
+  int n_row_tiles;
 };
 
 // ---- launchers (ekpnp_kernels.hip) ----------------------------------------
@@ -100,6 +103,12 @@ hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s);
 // PB-Picard start: Boltzmann concentrations from psi, then under-relaxation
 // of psi against psi_old (psi_old is updated to the relaxed value)
 hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s);
+// row means over x of the macroscopic rho, c+, c- into out ([3][ny] per channel)
+hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s);
+// re-express every stored population against new_refs (state in cur, or in
+// other for channels whose swap flag is set)
+hipError_t launch_rebase(int precision, const StepBuffers& b, void* cur, void* other, const double* new_refs,
+                         hipStream_t s);
 hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s);
 
 }  // namespace ekpnp

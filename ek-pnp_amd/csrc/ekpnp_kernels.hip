@@ -98,12 +98,25 @@ __device__ __forceinline__ void efield_at(const double* __restrict__ ps, int nx,
   Ey = (ps[(long long)ya * nx + x] - ps[(long long)yb * nx + x]) * inv2dy;
 }
 
+// Storage reference of a row: the populations stored at a node of row y are
+// deviations from w_i * (rho, c+, c-)_ref(y) (all zero for FP64 storage).
+// R points at the channel's [3][ny] reference rows.
+struct RowRef {
+  double r, p, n;
+};
+__device__ __forceinline__ RowRef row_ref(const double* __restrict__ R, int ny, int y) {
+  return RowRef{R[y], R[ny + y], R[2 * ny + y]};
+}
+
 // Pre-collision populations of node (x, y), 27 values: f 0..8, h 9..17,
-// g 18..26.  FROM_Q: the lattice holds them unshifted.  Otherwise gather the
-// post-collision values of the upstream neighbours (pull streaming).
+// g 18..26, as deviations from row y's reference.  FROM_Q: the lattice holds
+// them unshifted.  Otherwise gather the post-collision values of the upstream
+// neighbours (pull streaming); values streamed from rows y-1 and y+1 are
+// re-expressed against row y's reference.
 template <typename S, typename C, bool FROM_Q>
 __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plane, int nx, int ny,
-                                         int x, int y, int xm, int xp, C* q) {
+                                         int x, int y, int xm, int xp, const double* __restrict__ R,
+                                         C* q) {
   const long long r0 = (long long)y * nx;
   if (FROM_Q) {
 #pragma unroll
@@ -116,6 +129,8 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const S* B = P + (long long)(9 * s) * plane;
+    const double own = R[s * ny + y];
+    const C dm = C(R[s * ny + ym] - own), dp = C(R[s * ny + yp] - own);
     q[9 * s + 0] = (C)ld_pop(B + 0 * plane + r0 + x);
     q[9 * s + 1] = (C)ld_pop(B + 1 * plane + r0 + xm);
     q[9 * s + 2] = (C)ld_pop(B + 2 * plane + rm + x);
@@ -125,6 +140,12 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
     q[9 * s + 6] = (C)ld_pop(B + 6 * plane + rm + xp);
     q[9 * s + 7] = (C)ld_pop(B + 7 * plane + rp + xp);
     q[9 * s + 8] = (C)ld_pop(B + 8 * plane + rp + xm);
+    q[9 * s + 2] += C(WS) * dm;
+    q[9 * s + 5] += C(WD) * dm;
+    q[9 * s + 6] += C(WD) * dm;
+    q[9 * s + 4] += C(WS) * dp;
+    q[9 * s + 7] += C(WD) * dp;
+    q[9 * s + 8] += C(WD) * dp;
   }
 }
 
@@ -220,14 +241,14 @@ struct Moments {
 };
 
 template <typename C>
-__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, double Ex,
-                                              double Ey) {
+__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, const RowRef& rr,
+                                              double Ex, double Ey) {
   Moments<C> m;
   m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-  m.rho = C(cd->rho_ref) + m.dr;
+  m.rho = C(rr.r) + m.dr;
   m.dcp = ((q[9] + q[10]) + (q[11] + q[12])) + ((q[13] + q[14]) + (q[15] + q[16])) + q[17];
   m.dcn = ((q[18] + q[19]) + (q[20] + q[21])) + ((q[22] + q[23]) + (q[24] + q[25])) + q[26];
-  m.rhoe = (m.dcp - m.dcn) + C(cd->cp_ref - cd->cn_ref);
+  m.rhoe = (m.dcp - m.dcn) + C(rr.p - rr.n);
   const C jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
   const C jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
   m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
@@ -260,37 +281,39 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
   const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
   const double* __restrict__ ps = b.psi + cd->fld_off;
+  const double* __restrict__ R = b.refs + cd->ref_off;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
   const bool bottom = (y == 0), top = (y == ny - 1);
+  const RowRef rr = row_ref(R, ny, y);
 
   // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
   // included (LBM.cu:402-443); evaluated first so its registers are free.
   C mux = 0, muy = 0;
   if (bottom) {
     C m1[27];
-    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, m1);
+    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, R, m1);
     double Ex1, Ey1;
     efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
-    const Moments<C> mm = moments(m1, cd, Ex1, Ey1);
+    const Moments<C> mm = moments(m1, cd, row_ref(R, ny, 1), Ex1, Ey1);
     mux = -mm.ux;
     muy = -mm.uy;
   }
 
   C q[27];
-  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, q);
+  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, R, q);
   if (!FROM_Q && (bottom || top)) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - cd->cp_ref), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - cd->cn_ref), q + 18);
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.p), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.n), q + 18);
   }
 
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
-  const Moments<C> m = moments(q, cd, Ex, Ey);
+  const Moments<C> m = moments(q, cd, rr, Ex, Ey);
   C ux = m.ux, uy = m.uy;
   if (bottom) {
     ux = mux;
@@ -305,22 +328,22 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
       sp += (double)q[9 + k];
       sn += (double)q[18 + k];
     }
-    b.rho_e[cd->fld_off + node] = (sp - sn) + (cd->cp_ref - cd->cn_ref);
+    b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.p - rr.n);
   }
   if (MACRO) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
     b.macro[M_RHO * st + o] = (double)m.rho;
     b.macro[M_UX * st + o] = (double)ux;
     b.macro[M_UY * st + o] = (double)uy;
-    b.macro[M_CP * st + o] = (double)(C(cd->cp_ref) + m.dcp);
-    b.macro[M_CN * st + o] = (double)(C(cd->cn_ref) + m.dcn);
+    b.macro[M_CP * st + o] = (double)(C(rr.p) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(rr.n) + m.dcn);
   }
 
   const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
   // ions: drift velocity u + K E (LBM.cu:479-488)
   {
     const C Eix = C(Ex + cd->drift_app), Eiy = C(Ey);
-    const C cp = C(cd->cp_ref) + m.dcp, cn = C(cd->cn_ref) + m.dcn;
+    const C cp = C(rr.p) + m.dcp, cn = C(rr.n) + m.dcn;
     const C kp = C(cd->kp), kn = C(cd->kn);
     const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
     const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
@@ -379,7 +402,8 @@ __global__ __launch_bounds__(LBM_WG) void k_init_eq(StepBuffers b, S* __restrict
   const double vel[3][2] = {{ux, uy}, {ux + cd->kp * Ex, uy + cd->kp * Ey},
                             {ux + cd->kn * Ex, uy + cd->kn * Ey}};
   const double X[3] = {rho, cp, cn};
-  const double ref[3] = {cd->rho_ref, cd->cp_ref, cd->cn_ref};
+  const RowRef rr = row_ref(b.refs + cd->ref_off, ny, y);
+  const double ref[3] = {rr.r, rr.p, rr.n};
   S* __restrict__ Q = dst + cd->pop_off;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
@@ -413,15 +437,17 @@ __global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restr
   const long long plane = cd->plane;
   const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
+  const double* __restrict__ R = b.refs + cd->ref_off;
+  const RowRef rr = row_ref(R, ny, y);
   C q[27];
-  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, q);
+  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, R, q);
   if (y == 0 || y == ny - 1) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = (y == 0) ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = (y == 0) ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - cd->cp_ref), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - cd->cn_ref), q + 18);
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.p), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.n), q + 18);
   }
   S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
 #pragma unroll
@@ -541,6 +567,13 @@ __device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __
 
 __device__ __forceinline__ int ilog2(int v) { return 31 - __builtin_clz(v); }
 
+// Spectrum layout per channel: blocks of 8 wavenumbers, [k/8][y][k%8], so
+// that one wall-normal column block (8 modes x all rows) is contiguous for
+// the tridiagonal sweep and a row segment of 8 modes is one 128-byte line.
+__device__ __forceinline__ long long spec_idx(int ny, int y, int k) {
+  return ((long long)(k >> 3) * ny + y) * 8 + (k & 7);
+}
+
 // right-hand side of the lifted wall-normal system, scaled by dy^2
 // (odd_extension, poisson.cu:229-241)
 __device__ __forceinline__ double rhs_at(const double* __restrict__ re, const double* __restrict__ Wl,
@@ -593,14 +626,15 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
     __syncthreads();
     stockham<-1>(lds, rows, N, lgN, twq, lgnx);
     // split the packed transform: X[k] = E[k] + W^k O[k]
+    // rows vary fastest across lanes: 8 modes x `rows` rows are contiguous
     for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
-      const int r = e / (N + 1), k = e - r * (N + 1);
+      const int k = e / rows, r = e - k * rows;
       const double2 zk = lds[(r << lgN) + (k & (N - 1))];
       const double2 zc = cconj(lds[(r << lgN) + ((N - k) & (N - 1))]);
       const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
       const double2 dd = csub(zk, zc);
       const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
-      spec[(long long)(y0 + r) * cd->nk_stride + k] = cadd(E, cmul(tw_half(twq, k, quarter), O));
+      spec[spec_idx(cd->ny, y0 + r, k)] = cadd(E, cmul(tw_half(twq, k, quarter), O));
     }
   } else {
     // direct real DFT of one row
@@ -617,7 +651,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
         idx += k;
         if (idx >= nx) idx -= nx;
       }
-      spec[(long long)y0 * cd->nk_stride + k] = acc;
+      spec[spec_idx(cd->ny, y0, k)] = acc;
     }
   }
 }
@@ -655,12 +689,12 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
     }
     const double2* twq = TWL ? twl : tw;
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-      const int r = e >> lgN, k = e & (N - 1);
-      const double2* row = spec + (long long)(y0 + r) * cd->nk_stride;
-      const double2 xk = row[k], xc = cconj(row[N - k]);
+      const int k = e / rows, r = e - k * rows;
+      const double2 xk = spec[spec_idx(cd->ny, y0 + r, k)];
+      const double2 xc = cconj(spec[spec_idx(cd->ny, y0 + r, N - k)]);
       const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
       const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw_half(twq, k, quarter)));
-      lds[e] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
+      lds[(r << lgN) + k] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
     }
     __syncthreads();
     stockham<+1>(lds, rows, N, lgN, twq, lgnx);
@@ -672,7 +706,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
       *out = make_double2(z.x * invN, z.y * invN);
     }
   } else {
-    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[(long long)y0 * cd->nk_stride + k];
+    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[spec_idx(cd->ny, y0, k)];
     __syncthreads();
     const int kmax = (nx - 1) / 2;  // modes counted twice
     const bool nyq = (nx & 1) == 0;
@@ -712,6 +746,58 @@ __device__ __forceinline__ double cprime_closed(int i, double kd) {
   return -exp(-th) * expm1(-2.0 * (double)i * th) / expm1(-2.0 * (double)(i + 1) * th);
 }
 
+// Exclusive scan of affine maps v -> A v + B along the chunks of one mode.
+// FWD: chunk j receives the composition of chunks 0..j-1 applied to 0 (the
+// forward carry); otherwise chunks j+1..C-1 (the backward carry).  Lanes of
+// one mode sit G apart; a wave holds cpw chunks; sA/sB hold one total per
+// (wave, mode).
+template <bool FWD>
+__device__ __forceinline__ double2 chunk_scan(double A, double2 B, int G, int wv, int jw, int cpw, int m,
+                                              double* sA, double2* sB) {
+  // inclusive scan inside the wave
+  for (int d = 1; d < cpw; d <<= 1) {
+    const int off = d * G;
+    const double An = FWD ? __shfl_up(A, off, 64) : __shfl_down(A, off, 64);
+    const double Bx = FWD ? __shfl_up(B.x, off, 64) : __shfl_down(B.x, off, 64);
+    const double By = FWD ? __shfl_up(B.y, off, 64) : __shfl_down(B.y, off, 64);
+    const bool has = FWD ? (jw >= d) : (jw + d < cpw);
+    if (has) {
+      B = make_double2(A * Bx + B.x, A * By + B.y);
+      A = A * An;
+    }
+  }
+  // wave totals: the last chunk of the wave in scan order
+  const int nw = TRI_WG / 64;
+  if (jw == (FWD ? cpw - 1 : 0)) {
+    sA[wv * 64 + m] = A;
+    sB[wv * 64 + m] = B;
+  }
+  // this lane's exclusive value inside the wave
+  const int off = G;
+  const double Ae = FWD ? __shfl_up(A, off, 64) : __shfl_down(A, off, 64);
+  const double Bex = FWD ? __shfl_up(B.x, off, 64) : __shfl_down(B.x, off, 64);
+  const double Bey = FWD ? __shfl_up(B.y, off, 64) : __shfl_down(B.y, off, 64);
+  const bool first = FWD ? (jw == 0) : (jw == cpw - 1);
+  __syncthreads();
+  // carry entering this wave: the totals of the waves before it, in order
+  double2 Y = make_double2(0.0, 0.0);
+  if (FWD) {
+    for (int w = 0; w < wv; ++w) {
+      const double a = sA[w * 64 + m];
+      const double2 bb = sB[w * 64 + m];
+      Y = make_double2(a * Y.x + bb.x, a * Y.y + bb.y);
+    }
+  } else {
+    for (int w = nw - 1; w > wv; --w) {
+      const double a = sA[w * 64 + m];
+      const double2 bb = sB[w * 64 + m];
+      Y = make_double2(a * Y.x + bb.x, a * Y.y + bb.y);
+    }
+  }
+  if (first) return Y;
+  return make_double2(Ae * Y.x + Bex, Ae * Y.y + Bey);
+}
+
 template <int LMAX>
 __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   __shared__ double sA[TRI_WG];
@@ -719,8 +805,9 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   const int2 t = b.tri_tiles[blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
   if (!cd->active) return;
-  const int G = cd->tri_G, C = cd->tri_C, L = cd->tri_L;
-  const int nk = cd->nk, ks = cd->nk_stride, M = cd->ny - 2;
+  const int G = cd->tri_G, L = cd->tri_L;
+  const int nk = cd->nk, M = cd->ny - 2;
+  const int ks = 8;  // row stride inside a column block (spec_idx)
   const int tid = threadIdx.x;
   const int m = tid & (G - 1), j = tid / G;
   const int k = t.y + m;
@@ -728,7 +815,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   const int i0 = 1 + j * L;
   int len = M - j * L;
   len = len < 0 ? 0 : (len > L ? L : len);
-  double2* __restrict__ col = b.spec + cd->spec_off + (valid ? k : 0);
+  double2* __restrict__ col = b.spec + cd->spec_off + (valid ? spec_idx(cd->ny, 0, k) : 0);
   const double kd = (double)k * cd->kdy_scale;
   const double a = -(2.0 + kd * kd);
 
@@ -757,29 +844,11 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
     yi[ii] = pim;
     A = on ? -c * A : A;
   }
-  sA[tid] = A;
-  sB[tid] = make_double2(pr, pim);
-  __syncthreads();
-  // inclusive prefix scan over chunks: S_j = M_j o S_{j-1}
-  for (int d = 1; d < C; d <<= 1) {
-    const bool has = j >= d;
-    double Ap = 1.0;
-    double2 Bp = make_double2(0.0, 0.0);
-    if (has) {
-      Ap = sA[tid - d * G];
-      Bp = sB[tid - d * G];
-    }
-    const double Ac = sA[tid];
-    const double2 Bc = sB[tid];
-    __syncthreads();
-    if (has) {
-      sA[tid] = Ac * Ap;
-      sB[tid] = make_double2(Ac * Bp.x + Bc.x, Ac * Bp.y + Bc.y);
-    }
-    __syncthreads();
-  }
-  const double2 Yin = (j > 0) ? sB[tid - G] : make_double2(0.0, 0.0);
-  __syncthreads();
+  // Exclusive scan of the chunk maps y -> A y + B over j: shuffles inside a
+  // wave (the wave holds 64/G consecutive chunks of its G modes), one LDS
+  // exchange of wave totals across the work-group.
+  const int wv = tid >> 6, jw = (tid & 63) / G, cpw = 64 / G;
+  const double2 Yin = chunk_scan<true>(A, make_double2(pr, pim), G, wv, jw, cpw, m, sA, sB);
   // carry into the forward sweep, then local backward sweep with x_{e+1} = 0
   // (masked rows have cc = 0: the carry product becomes 0 there, and the
   // backward recurrence passes x through unchanged only for rows < len)
@@ -801,28 +870,8 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
     yr[ii] = xr;
     yi[ii] = xim;
   }
-  sA[tid] = A;
-  sB[tid] = make_double2(xr, xim);
-  __syncthreads();
-  // inclusive suffix scan: T_j = N_j o T_{j+1}
-  for (int d = 1; d < C; d <<= 1) {
-    const bool has = j + d < C;
-    double An = 1.0;
-    double2 Bn = make_double2(0.0, 0.0);
-    if (has) {
-      An = sA[tid + d * G];
-      Bn = sB[tid + d * G];
-    }
-    const double Ac = sA[tid];
-    const double2 Bc = sB[tid];
-    __syncthreads();
-    if (has) {
-      sA[tid] = Ac * An;
-      sB[tid] = make_double2(Ac * Bn.x + Bc.x, Ac * Bn.y + Bc.y);
-    }
-    __syncthreads();
-  }
-  const double2 Xin = (j + 1 < C) ? sB[tid + G] : make_double2(0.0, 0.0);
+  __syncthreads();  // the forward exchange's LDS is reused
+  const double2 Xin = chunk_scan<false>(A, make_double2(xr, xim), G, wv, jw, cpw, m, sA, sB);
   P = 1.0;
 #pragma unroll
   for (int ii = LMAX - 1; ii >= 0; --ii) {
@@ -842,6 +891,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
 // FP64 reductions per channel: partial sums per LBM tile (wave shuffles +
 // LDS), then one work-group per channel adds its tiles in a fixed order.
 // ---------------------------------------------------------------------------
+// FP64 sum over the 64 lanes of a wave, result in lane 0
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -906,6 +956,61 @@ __global__ __launch_bounds__(256) void k_reduce_final(StepBuffers b) {
   if (threadIdx.x < 8)
     b.red_out[c * 8 + threadIdx.x] = (red[0][threadIdx.x] + red[1][threadIdx.x]) +
                                      (red[2][threadIdx.x] + red[3][threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+
+// Storage references: row means of rho, c+, c- over x (one work-group per
+// channel row), and the exact change of reference of every stored population
+// (delta += w_i (ref_old - ref_new), in FP64, rounded once).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_row_means(StepBuffers b, double* __restrict__ out) {
+  __shared__ double red[4][3];
+  const int2 t = b.row_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int nx = cd->nx, ny = cd->ny, y = t.y;
+  const long long o = cd->fld_off + (long long)y * nx, st = b.macro_stride;
+  double v[3] = {0.0, 0.0, 0.0};
+  for (int x = threadIdx.x; x < nx; x += 256) {
+    v[0] += b.macro[M_RHO * st + o + x];
+    v[1] += b.macro[M_CP * st + o + x];
+    v[2] += b.macro[M_CN * st + o + x];
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double s = wave_sum(v[k]);
+    if (lane == 0) red[wv][k] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const double s = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+    out[cd->ref_off + (long long)threadIdx.x * ny + y] = s / (double)nx;
+  }
+}
+
+template <typename S>
+__global__ __launch_bounds__(LBM_WG) void k_rebase(StepBuffers b, S* __restrict__ cur, S* __restrict__ other,
+                                                   const double* __restrict__ new_refs) {
+  const int2 tile = b.lbm_tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node = tile.y + (int)threadIdx.x;
+  if (node >= nx * ny) return;
+  const int y = node / nx;
+  S* __restrict__ P = (cd->swap ? other : cur) + cd->pop_off;
+  const double* __restrict__ R0 = b.refs + cd->ref_off;
+  const double* __restrict__ R1 = new_refs + cd->ref_off;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const double d = R0[s * ny + y] - R1[s * ny + y];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const double w = (k == 0) ? W0 : (k < 5 ? WS : WD);
+      S* p = P + (long long)(9 * s + k) * cd->plane + node;
+      *p = (S)((double)*p + w * d);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1016,6 +1121,23 @@ hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, voi
   else
     hipLaunchKernelGGL((k_to_q<double>), grid, block, 0, s, b, static_cast<const double*>(src),
                        static_cast<double*>(dst));
+  return hipGetLastError();
+}
+
+hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_means, dim3(b.n_row_tiles), dim3(256), 0, s, b, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_rebase(int precision, const StepBuffers& b, void* cur, void* other, const double* new_refs,
+                         hipStream_t s) {
+  const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
+  if (precision == 0)
+    hipLaunchKernelGGL((k_rebase<float>), grid, block, 0, s, b, static_cast<float*>(cur),
+                       static_cast<float*>(other), new_refs);
+  else
+    hipLaunchKernelGGL((k_rebase<double>), grid, block, 0, s, b, static_cast<double*>(cur),
+                       static_cast<double*>(other), new_refs);
   return hipGetLastError();
 }
 

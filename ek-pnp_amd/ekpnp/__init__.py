@@ -30,7 +30,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("EKPNP_LIB", os.path.join(os.path.dirname(_HERE), "libekpnp.so"))
 
 # status codes (include/ekpnp.h)
-OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED, ERR_UNSTABLE = 0, -1, -2, -3, -4, -5, -6
+OK, ERR_INVALID, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_UNSUPPORTED, ERR_UNSTABLE, ERR_IO = 0, -1, -2, -3, -4, -5, -6, -7
 POP_FP32, POP_FP64 = 0, 1
 FLAG_ION_DRIFT_APPLIED = 0x1
 FLAG_G0_FLUID_U = 0x2
@@ -95,6 +95,7 @@ SIGNATURES = [
     ("ekpnp_stream_collide_save", ctypes.c_int, [_VP]),
     ("ekpnp_fast_poisson", ctypes.c_int, [_VP]),
     ("ekpnp_step", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("ekpnp_rebase", ctypes.c_int, [_VP]),
     ("ekpnp_run_to_residual", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                              _P(ctypes.c_int)]),
     ("ekpnp_get_field", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
@@ -102,6 +103,15 @@ SIGNATURES = [
     ("ekpnp_set_populations", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
     ("ekpnp_reductions", ctypes.c_int, [_VP, _DP]),
     ("ekpnp_synchronize", ctypes.c_int, [_VP]),
+    ("ekpnp_save_data_end", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p]),
+    ("ekpnp_read_data", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_char_p, _DP]),
+    ("ekpnp_save_tecplot", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p, ctypes.c_int]),
+    ("ekpnp_save_charge_data", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_char_p]),
+    ("ekpnp_current", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
+    ("ekpnp_record_umax", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p, _DP]),
+    ("ekpnp_save_checkpoint", ctypes.c_int, [_VP, ctypes.c_char_p]),
+    ("ekpnp_load_checkpoint", ctypes.c_int, [_VP, ctypes.c_char_p]),
+    ("ekpnp_get_params", ctypes.c_int, [_VP, ctypes.c_int, _P(ChannelParams)]),
     ("ekpnp_num_channels", ctypes.c_int, [_VP]),
     ("ekpnp_steps_done", ctypes.c_longlong, [_VP]),
     ("ekpnp_device_bytes", ctypes.c_size_t, [_VP]),
@@ -290,6 +300,10 @@ class Solver:
     def step(self, n: int = 1) -> None:
         self._check(lib().ekpnp_step(self._ctx, int(n)))
 
+    def rebase(self) -> None:
+        """Re-centre FP32 storage on the current row means (no-op for FP64)."""
+        self._check(lib().ekpnp_rebase(self._ctx))
+
     def run_to_residual(self, tol: float = 1e-8, check_every: int = 100, max_steps: int = 10 ** 6):
         out = (ctypes.c_int * len(self.channels))()
         self._check(lib().ekpnp_run_to_residual(self._ctx, tol, check_every, max_steps, out))
@@ -320,6 +334,43 @@ class Solver:
         self._check(lib().ekpnp_reductions(self._ctx, _dptr(out)))
         return out
 
+    # -- files and diagnostics ----------------------------------------------
+    def save_data_end(self, ch: int, time: float, path: str) -> None:
+        self._check(lib().ekpnp_save_data_end(self._ctx, ch, float(time), os.fsencode(path)))
+
+    def read_data(self, ch: int, path: str) -> float:
+        t = ctypes.c_double(0.0)
+        self._check(lib().ekpnp_read_data(self._ctx, ch, os.fsencode(path), ctypes.byref(t)))
+        return t.value
+
+    def save_tecplot(self, ch: int, time: float, path: str, first: bool) -> None:
+        self._check(lib().ekpnp_save_tecplot(self._ctx, ch, float(time), os.fsencode(path), int(bool(first))))
+
+    def save_charge_data(self, ch: int, path: str) -> None:
+        self._check(lib().ekpnp_save_charge_data(self._ctx, ch, os.fsencode(path)))
+
+    def current(self, ch: int) -> float:
+        out = ctypes.c_double(0.0)
+        self._check(lib().ekpnp_current(self._ctx, ch, ctypes.byref(out)))
+        return out.value
+
+    def record_umax(self, ch: int, time: float = 0.0, path: str = None) -> float:
+        out = ctypes.c_double(0.0)
+        p = None if path is None else os.fsencode(path)
+        self._check(lib().ekpnp_record_umax(self._ctx, ch, float(time), p, ctypes.byref(out)))
+        return out.value
+
+    def save_checkpoint(self, path: str) -> None:
+        self._check(lib().ekpnp_save_checkpoint(self._ctx, os.fsencode(path)))
+
+    def load_checkpoint(self, path: str) -> None:
+        self._check(lib().ekpnp_load_checkpoint(self._ctx, os.fsencode(path)))
+
+    def get_params(self, ch: int) -> ChannelParams:
+        out = ChannelParams()
+        self._check(lib().ekpnp_get_params(self._ctx, ch, ctypes.byref(out)))
+        return out
+
     @property
     def steps_done(self) -> int:
         return int(lib().ekpnp_steps_done(self._ctx))
@@ -335,6 +386,7 @@ class Solver:
 
 
 __all__ = [
+    "OK", "ERR_INVALID", "ERR_HIP", "ERR_NOMEM", "ERR_STATE", "ERR_UNSUPPORTED", "ERR_UNSTABLE", "ERR_IO",
     "ChannelParams", "Options", "Solver", "EkpnpError", "lib", "reference_params", "lattice_params",
     "FIELDS", "SPECIES", "RED_NAMES", "COLD_NOISE", "REFERENCE", "PB_PICARD", "POP_FP32", "POP_FP64",
     "FLAG_ION_DRIFT_APPLIED", "FLAG_G0_FLUID_U", "SIGNATURES", "LIB_PATH",

@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Time-to-steady with surrogate warm starts versus cold starts.
+
+1. Training data: seeded config-2-style channels (zeta ~ U[-4,-0.4], kappa h ~
+   logU[1,50], E_x ~ logU[1e-5,1e-3], draws failing the resolution/stability
+   rule redrawn) are run by the batched solver from a cold start to the
+   u_x residual tolerance; their x-averaged steady profiles of psi, c+, c-,
+   u_x are the targets.  The steady state of an x-uniform channel does not
+   depend on nx, so training channels use a narrow nx.
+2. The MLP (4 x 256 GELU, BF16 MFMA kernels) is trained on the GPU.
+3. Held-out channels (another seed, nx from the config-2 set) are run twice:
+   cold (uniform c0 + noise) and warm (predicted profiles broadcast in x).
+   Every warm channel first runs a probe of `probe` steps; a channel whose
+   residual after the probe exceeds `accept` (or is not finite) is rejected
+   and restarted cold.  Both arms finish with the plain solver to the same
+   residual; step counts include the probe.
+
+Prints one JSON line with per-channel steps and the summary.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+import ekpnp  # noqa: E402
+from ekpnp import surrogate as S  # noqa: E402
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def xavg(s, ch, name):
+    return s.get_field(ch, name).mean(axis=1)
+
+
+def run_cold(channels, precision, seed, tol, every, max_steps):
+    s = ekpnp.Solver(channels, precision=precision)
+    s.initialization(ekpnp.COLD_NOISE, seed)
+    s.init_equilibrium()
+    s.synchronize()
+    t0 = time.perf_counter()
+    steps = s.run_to_residual(tol, every, max_steps)
+    s.synchronize()
+    return s, steps, time.perf_counter() - t0
+
+
+def generate(channels, precision, seed, tol, every, max_steps):
+    s, steps, wall = run_cold(channels, precision, seed, tol, every, max_steps)
+    X = S.channel_features(channels)
+    T = np.stack([S.targets(p, xavg(s, c, "psi"), xavg(s, c, "cpos"), xavg(s, c, "cneg"), xavg(s, c, "ux"))
+                  for c, p in enumerate(channels)])
+    s.close()
+    return X, T, steps, wall
+
+
+def train(X, T, epochs, batch, lr0, seed):
+    rng = np.random.default_rng(seed)
+    n = X.shape[0]
+    nval = max(1, n // 10)
+    perm = rng.permutation(n)
+    val, tr = perm[:nval], perm[nval:]
+    net = S.MLP(max_batch=batch, seed=seed)
+    steps_per_epoch = max(1, len(tr) // batch)
+    total = epochs * steps_per_epoch
+    it = 0
+    hist = []
+    for ep in range(epochs):
+        order = rng.permutation(tr)
+        for k in range(steps_per_epoch):
+            idx = order[k * batch:(k + 1) * batch]
+            lr = 1e-5 + 0.5 * (lr0 - 1e-5) * (1 + np.cos(np.pi * it / total))
+            net.train_step(X[idx], T[idx], lr)
+            it += 1
+        if ep % max(1, epochs // 10) == 0 or ep == epochs - 1:
+            yv = net.forward(X[val])
+            vl = float(np.mean((yv - T[val]) ** 2))
+            yt = net.forward(X[tr[:512]])
+            tl = float(np.mean((yt - T[tr[:512]]) ** 2))
+            hist.append((ep, tl, vl))
+            log(f"epoch {ep}: train mse {tl:.3e}  val mse {vl:.3e}")
+    return net, hist
+
+
+def run_warm(channels, net, precision, seed, tol, every, max_steps, probe, accept, corrupt=None):
+    """Warm arm; corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
+    t0 = time.perf_counter()
+    s = ekpnp.Solver(channels, precision=precision)
+    s.initialization(ekpnp.COLD_NOISE, seed)
+    cold = [{f: s.get_field(c, f) for f in ("rho", "ux", "uy", "cpos", "cneg")} for c in range(len(channels))]
+    pred = net.forward(S.channel_features(channels))
+    for c, p in enumerate(channels):
+        psi, cp, cn, ux = S.profiles(p, pred[c])
+        if corrupt is not None:
+            psi, cp, cn, ux = corrupt(c, psi, cp, cn, ux)
+        ny, nx = p.ny, p.nx
+        b = lambda v: np.repeat(v[:, None], nx, axis=1)  # noqa: E731
+        s.set_fields(c, np.full((ny, nx), p.rho0), b(ux), np.zeros((ny, nx)), b(cp), b(cn), b(psi))
+    s.init_equilibrium()
+    red0 = s.reductions()  # snapshot
+    s.step(probe)
+    red = s.reductions()
+    res = red[:, 7]
+    rejected = [c for c in range(len(channels)) if not (np.isfinite(res[c]) and res[c] <= accept)
+                or red[c, 6] > 0]
+    for c in rejected:
+        f = cold[c]
+        s.set_fields(c, f["rho"], f["ux"], f["uy"], f["cpos"], f["cneg"])
+    if rejected:
+        s.init_equilibrium()
+    steps = s.run_to_residual(tol, every, max_steps)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    total = np.where(steps >= 0, steps + probe, -1)
+    del red0
+    return s, total, wall, res, rejected
+
+
+def summarize(cold, warm):
+    ok = (cold > 0) & (warm > 0)
+    ratio = cold[ok] / warm[ok]
+    return {
+        "channels": int(len(cold)),
+        "both_converged": int(ok.sum()),
+        "cold_steps_median": float(np.median(cold[ok])) if ok.any() else None,
+        "warm_steps_median": float(np.median(warm[ok])) if ok.any() else None,
+        "step_ratio_median": float(np.median(ratio)) if ok.any() else None,
+        "step_ratio_geomean": float(np.exp(np.mean(np.log(ratio)))) if ok.any() else None,
+        "makespan_cold": int(cold.max()),
+        "makespan_warm": int(warm.max()),
+        "makespan_ratio": float(cold.max() / max(1, warm.max())),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=512)
+    ap.add_argument("--heldout", type=int, default=64)
+    ap.add_argument("--ny", default="64,128,192,256")
+    ap.add_argument("--nx-train", type=int, default=4)
+    ap.add_argument("--nx", default="32,64,128,256")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--every", type=int, default=100)
+    ap.add_argument("--max-steps", type=int, default=3_000_000)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--probe", type=int, default=200)
+    ap.add_argument("--accept", type=float, default=1e-4)
+    ap.add_argument("--precision", default="fp32")
+    ap.add_argument("--seed", type=int, default=1000)
+    ap.add_argument("--save", default="")
+    args = ap.parse_args()
+    ny = tuple(int(v) for v in args.ny.split(","))
+    nxs = tuple(int(v) for v in args.nx.split(","))
+
+    tr_ch, rej_tr = S.sample_channels(args.train, args.seed, nx_choices=(args.nx_train,), ny_choices=ny)
+    log(f"training channels: {len(tr_ch)} (redrawn {rej_tr})")
+    X, T, tr_steps, tr_wall = generate(tr_ch, args.precision, args.seed, args.tol, args.every, args.max_steps)
+    good = tr_steps > 0
+    log(f"data: {good.sum()} converged, steps median {np.median(tr_steps[good]):.0f}, wall {tr_wall:.1f} s")
+    t0 = time.perf_counter()
+    net, hist = train(X[good], T[good], args.epochs, args.batch, args.lr, args.seed)
+    train_wall = time.perf_counter() - t0
+
+    ho_ch, rej_ho = S.sample_channels(args.heldout, 900000 + args.seed, nx_choices=nxs, ny_choices=ny)
+    s, cold, cold_wall = run_cold(ho_ch, args.precision, 900000 + args.seed, args.tol, args.every, args.max_steps)
+    s.close()
+    log(f"cold: median steps {np.median(cold):.0f}, wall {cold_wall:.1f} s")
+    s, warm, warm_wall, probe_res, rejected = run_warm(ho_ch, net, args.precision, 900000 + args.seed, args.tol,
+                                                      args.every, args.max_steps, args.probe, args.accept)
+    s.close()
+    log(f"warm: median steps {np.median(warm):.0f}, wall {warm_wall:.1f} s, rejected {len(rejected)}")
+    out = {
+        "experiment": "time-to-steady, surrogate-warm vs cold",
+        "tol": args.tol, "check_every": args.every, "precision": args.precision,
+        "train": {"channels": len(tr_ch), "redrawn": rej_tr, "converged": int(good.sum()),
+                  "wall_s": round(tr_wall, 2), "train_wall_s": round(train_wall, 2), "history": hist},
+        "heldout": {"channels": len(ho_ch), "redrawn": rej_ho, "nx": list(nxs), "ny": list(ny),
+                    "cold_wall_s": round(cold_wall, 2), "warm_wall_s": round(warm_wall, 2),
+                    "wall_ratio": round(cold_wall / warm_wall, 3), "rejected": rejected,
+                    "probe_residual_median": float(np.median(probe_res)),
+                    "cold_steps": cold.tolist(), "warm_steps": warm.tolist(),
+                    "summary": summarize(cold, warm)},
+    }
+    if args.save:
+        np.savez(args.save, params=net.get_params(), X=X, T=T)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

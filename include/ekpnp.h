@@ -23,6 +23,16 @@
  *   ekpnp_get_field         LBM.cu:1272-1312 (the D2H copies of save_data_end)
  *   ekpnp_get_populations / ekpnp_set_populations
  *                           f0/f1, h0/h1, temp0/temp1 device arrays, LBM.h:124-126
+ *   ekpnp_save_data_end     LBM.cu:1272-1312 (save_data_end, data_end.dat)
+ *   ekpnp_read_data         LBM.cu:1350-1391 (read_data, the flag == 1 restart)
+ *   ekpnp_save_tecplot      LBM.cu:1100-1269 (save_data_tecplot, data.dat)
+ *   ekpnp_save_charge_data  LBM.cu:1314-1348 (save_data_dmd, charge_data)
+ *   ekpnp_current           LBM.cu:1394-1417 (current)
+ *   ekpnp_record_umax       LBM.cu:1419-1451 (record_umax, umax.dat)
+ *   ekpnp_save_checkpoint / ekpnp_load_checkpoint
+ *                           no counterpart: exact binary restart (the
+ *                           reference's data_end.dat restart re-equilibrates
+ *                           from 6-decimal fields and loses the populations)
  */
 #ifndef EKPNP_H
 #define EKPNP_H
@@ -42,12 +52,13 @@ enum {
   EKPNP_ERR_NOMEM = -3,       /* device or host allocation failed           */
   EKPNP_ERR_STATE = -4,       /* entry point called out of order            */
   EKPNP_ERR_UNSUPPORTED = -5, /* shape outside the kernels' limits          */
-  EKPNP_ERR_UNSTABLE = -6     /* NaN / runaway found by the reductions      */
+  EKPNP_ERR_UNSTABLE = -6,    /* NaN / runaway found by the reductions      */
+  EKPNP_ERR_IO = -7           /* file could not be opened, read or written  */
 };
 
 /* ---- population storage ------------------------------------------------ */
 enum {
-  EKPNP_POP_FP32 = 0, /* FP32 deviations from w_i*rho0, w_i*c_bulk         */
+  EKPNP_POP_FP32 = 0, /* FP32 deviations from w_i*(rho, c+, c-)_ref(y)    */
   EKPNP_POP_FP64 = 1  /* FP64 full populations (the reference's storage)   */
 };
 
@@ -188,10 +199,20 @@ int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho,
                      const double* cneg, const double* psi);
 int ekpnp_init_equilibrium(ekpnp_ctx* ctx);
 
+#define EKPNP_REBASE_INTERVAL 4096
+
 /* ---- the coupled step ------------------------------------------------------ */
 int ekpnp_stream_collide_save(ekpnp_ctx* ctx);
 int ekpnp_fast_poisson(ekpnp_ctx* ctx);
+/* With FP32 storage, ekpnp_step re-centres the storage reference every
+ * EKPNP_REBASE_INTERVAL steps (see ekpnp_rebase). */
 int ekpnp_step(ekpnp_ctx* ctx, int nsteps);
+/* FP32 storage keeps each population as a deviation from w_i times a per-row
+ * reference (rho, c+, c-)(y).  This re-centres the references on the current
+ * row means over x; the state it represents is unchanged up to one FP32
+ * rounding.  No-op for FP64 storage.  No counterpart in the reference (its
+ * populations are absolute FP64 values). */
+int ekpnp_rebase(ekpnp_ctx* ctx);
 /* Steps until every channel's residual sqrt(sum (ux-ux_prev)^2 / sum ux^2)
  * / check_every over interior rows is <= tol, or max_steps.  check_every
  * must be even.  steps_out[channel] receives the step count at which each
@@ -212,7 +233,34 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species,
 int ekpnp_reductions(ekpnp_ctx* ctx, double* out);
 int ekpnp_synchronize(ekpnp_ctx* ctx);
 
+/* ---- files and diagnostics (host formatting of one channel's fields) -------- */
+/* data_end.dat: one line per node (y outer, x inner), "%10.6f" x 9:
+ * t, ux, uy, rho, c+, psi, Ex, Ey, c-. */
+int ekpnp_save_data_end(ekpnp_ctx* ctx, int channel, double time, const char* path);
+/* Reads a data_end.dat written for this channel's nx, ny into the fields
+ * (like ekpnp_set_fields with psi; E is recomputed from psi).  *time_out
+ * (may be NULL) receives the file's time.  Follow with init_equilibrium. */
+int ekpnp_read_data(ekpnp_ctx* ctx, int channel, const char* path, double* time_out);
+/* Tecplot POINT zone of 14 variables; first != 0 truncates the file and
+ * writes the VARIABLES line, else the zone is appended. */
+int ekpnp_save_tecplot(ekpnp_ctx* ctx, int channel, double time, const char* path, int first);
+/* Appends c+ (top row extrapolated), one "%10.6f " value per line. */
+int ekpnp_save_charge_data(ekpnp_ctx* ctx, int channel, const char* path);
+/* Wall current K dy sum_x c+(x, ny-1) Ey(x, ny-1), c+ extrapolated. */
+int ekpnp_current(ekpnp_ctx* ctx, int channel, double* out);
+/* max(0, max u_y) with the top row extrapolated; appended to path as
+ * "%10.6f %10.6f\n" (time, umax) unless path is NULL. */
+int ekpnp_record_umax(ekpnp_ctx* ctx, int channel, double time, const char* path, double* umax_out);
+/* Exact restart: every channel's stored populations (in the storage
+ * precision), storage references, psi, macroscopic fields, wall profiles,
+ * residual snapshot and step counters.  load needs a context created with
+ * the same channel shapes and precision. */
+int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path);
+int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
+
 /* ---- introspection / timing -------------------------------------------------- */
+/* The parameters channel was created with (wall profiles not included). */
+int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out);
 int ekpnp_num_channels(const ekpnp_ctx* ctx);
 long long ekpnp_steps_done(const ekpnp_ctx* ctx);
 /* Device bytes held by the context. */
