@@ -41,6 +41,12 @@ def workload(name):
     if name == "c0":
         p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
         return [p], "fp64", 0, "single channel 64x128, tau 0.8, zeta -1, kappa_h 8, E_x 1e-4, FP64"
+    if name == "c2":
+        from ekpnp import surrogate as S
+
+        ps, _ = S.sample_channels(256, 2)
+        return ps, "fp32", 2, ("batched sweep: 256 channels, seed 2, nx ~ U{32,64,128,256}, ny ~ U{64..1024 "
+                               "step 64}, zeta ~ U[-4,-0.4], kappa_h ~ logU[1,50], E_x ~ logU[1e-5,1e-3]")
     if name == "c4":
         p = ekpnp.lattice_params(8192, 2048, tau=0.8, zeta=-1.5, kappa_h=64.0, e_x=1e-4)
         return [p], "fp32", 4, ("single channel 8192x2048, wall zeta(x) = -1.5 + sum_k a_k sin(2 pi k x/nx "
@@ -130,11 +136,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--config", default="c1")
+    ap.add_argument("--config", default="c1", choices=("c0", "c1", "c2", "c4"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--stage-steps", type=int, default=20)
     ap.add_argument("--compute", choices=("fp32", "fp64"), default="fp64",
                     help="collision arithmetic for FP32 population storage")
+    ap.add_argument("--tune-reps", type=int, default=10,
+                    help="Poisson solves per candidate plan in the autotuner (0: default plan)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -154,6 +162,7 @@ def main():
     print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     s = ekpnp.Solver(params, precision=precision, compute_fp64=(args.compute == "fp64"))
+    plan = s.autotune(args.tune_reps) if args.tune_reps > 0 else None
     apply_walls(s, args.config, params, seed)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
@@ -187,7 +196,8 @@ def main():
         "dtype": "f32 populations, f64 Poisson" if precision == "fp32" else "f64",
         "data": "synthetic (seeded cold start: c0 = 1 with U(-1e-3, 1e-3) noise)",
         "config": {"workload": desc, "nodes": nodes, "precision": precision,
-                   "collision_arithmetic": "f64" if (precision == "fp64" or args.compute == "fp64") else "f32"},
+                   "collision_arithmetic": "f64" if (precision == "fp64" or args.compute == "fp64") else "f32",
+                   "poisson_plan": plan},
         "hbm_frac_coupled": round(mlups * 1e6 * coupled_b / (HBM_PEAK_GBS * 1e9), 4),
         "stages_ms": {k: round(v, 5) for k, v in st.items()},
         "roofline": {

@@ -88,6 +88,10 @@ struct ekpnp_ctx {
   long long steps = 0;
   long long snapshot_step = -1;
   long long rebase_step = 0;  // step count at the last change of storage reference
+  // Poisson work decomposition (see build_poisson_plan)
+  int fft_elems = 2048, tri_rows = 16;
+  size_t fft_tiles_cap = 0, tri_tiles_cap = 0;
+  unsigned max_lds_needed = 0;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
   std::string err;
 };
@@ -172,13 +176,6 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.c_bulk = p.c_bulk;
   d.inv_vt = p.inv_thermal_voltage;
   d.pb_omega = p.pb_omega;
-  // tridiagonal partition: C chunks of L rows, G = TRI_WG / C modes per WG
-  const int M = p.ny - 2;
-  int C = pow2_ceil((M + 15) / 16);
-  C = std::max(8, std::min(128, C));
-  d.tri_C = C;
-  d.tri_L = (M + C - 1) / C;
-  d.tri_G = TRI_WG / C;
   return d;
 }
 
@@ -218,9 +215,10 @@ hipError_t dalloc(ekpnp_ctx* c, T** p, size_t count) {
   return e;
 }
 
+void drop_graphs(ekpnp_ctx* c);
+
 void release(ekpnp_ctx* c) {
-  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
-  c->graphs.clear();
+  drop_graphs(c);
   void* ptrs[] = {c->lat[0], c->lat[1], c->d_desc, c->d_lbm_tiles, c->d_fft_tiles,
                   c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
                   c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red,
@@ -229,6 +227,83 @@ void release(ekpnp_ctx* c) {
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   c->stream = nullptr;
+}
+
+void drop_graphs(ekpnp_ctx* c) {
+  for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
+  c->graphs.clear();
+}
+
+// Poisson work decomposition.  x-FFT: a work-group transforms whole rows,
+// up to fft_elems complex values (power-of-two channels; the direct DFT of
+// other widths takes one row per work-group); power-of-two tiles come first
+// in the table.  Tridiagonal: each column is cut into C = 2^k chunks of at
+// most about tri_rows rows (8 <= C <= 128), G = TRI_WG / C modes per
+// work-group.  Rebuilding drops the cached graphs (their grids change).
+hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows) {
+  c->fft_tiles.clear();
+  c->tri_tiles.clear();
+  unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;
+  int n_fft_pow2 = 0;
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int i = 0; i < c->nchan; ++i) {
+      ChanDesc& d = c->desc[i];
+      if ((pass == 0) != (d.pow2 == 1)) continue;
+      const int M = d.ny - 2;
+      int rows = 1;
+      if (d.pow2) {
+        const int N = d.nx / 2;
+        rows = std::max(1, std::min(M, fft_elems / N));
+        lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(rows * N * sizeof(double2)));
+        lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((rows * N + d.nx / 4) * sizeof(double2)));
+      } else {
+        lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
+      }
+      d.fft_rows = rows;
+      c->ctiles[i].fft_first = (int)c->fft_tiles.size();
+      for (int y0 = 1; y0 <= M; y0 += rows)
+        c->fft_tiles.push_back(make_int4(i, y0, std::min(rows, M + 1 - y0), 0));
+      c->ctiles[i].fft_count = (int)c->fft_tiles.size() - c->ctiles[i].fft_first;
+    }
+    if (pass == 0) n_fft_pow2 = (int)c->fft_tiles.size();
+  }
+  int tri_lmax = 16;
+  for (int i = 0; i < c->nchan; ++i) {
+    ChanDesc& d = c->desc[i];
+    const int M = d.ny - 2;
+    int C = pow2_ceil((M + tri_rows - 1) / tri_rows);
+    C = std::max(8, std::min(128, C));
+    d.tri_C = C;
+    d.tri_L = (M + C - 1) / C;
+    d.tri_G = TRI_WG / C;
+    c->ctiles[i].tri_first = (int)c->tri_tiles.size();
+    for (int k0 = 0; k0 < d.nk; k0 += d.tri_G) c->tri_tiles.push_back(make_int2(i, k0));
+    c->ctiles[i].tri_count = (int)c->tri_tiles.size() - c->ctiles[i].tri_first;
+    if (d.tri_L > 16) tri_lmax = 32;
+  }
+  if (c->fft_tiles.size() > c->fft_tiles_cap || c->tri_tiles.size() > c->tri_tiles_cap) return hipErrorInvalidValue;
+  hipError_t e = hipMemcpy(c->d_fft_tiles, c->fft_tiles.data(), c->fft_tiles.size() * sizeof(int4),
+                           hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(c->d_tri_tiles, c->tri_tiles.data(), c->tri_tiles.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  e = hipMemcpy(c->d_desc, c->desc.data(), c->nchan * sizeof(ChanDesc), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  c->sb.fft_tiles = c->d_fft_tiles;
+  c->sb.n_fft_tiles = (int)c->fft_tiles.size();
+  c->sb.n_fft_pow2 = n_fft_pow2;
+  c->sb.tri_tiles = c->d_tri_tiles;
+  c->sb.n_tri_tiles = (int)c->tri_tiles.size();
+  // rows + quadrant twiddle table in LDS when the device allows that much
+  // dynamic LDS per work-group; otherwise twiddles are read from global memory
+  c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)c->max_lds) ? 1 : 0;
+  c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
+  c->sb.fft_lds_dft = lds_dft;
+  c->sb.tri_lmax = tri_lmax;
+  c->fft_elems = fft_elems;
+  c->tri_rows = tri_rows;
+  drop_graphs(c);
+  return hipSuccess;
 }
 
 // Poisson stages for all channels, or for one channel's tile ranges.
@@ -486,38 +561,11 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
-  // FFT tiles: power-of-two channels first
-  unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;
-  for (int pass = 0; pass < 2; ++pass) {
-    for (int i = 0; i < n_channels; ++i) {
-      ChanDesc& d = c->desc[i];
-      if ((pass == 0) != (d.pow2 == 1)) continue;
-      const int M = d.ny - 2;
-      int rows = 1;
-      if (d.pow2) {
-        const int N = d.nx / 2;
-        rows = std::max(1, std::min(M, FFT_MAX_ELEMS / N));
-        lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(rows * N * sizeof(double2)));
-        lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((rows * N + d.nx / 4) * sizeof(double2)));
-      } else {
-        lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
-      }
-      d.fft_rows = rows;
-      c->ctiles[i].fft_first = (int)c->fft_tiles.size();
-      for (int y0 = 1; y0 <= M; y0 += rows)
-        c->fft_tiles.push_back(make_int4(i, y0, std::min(rows, M + 1 - y0), 0));
-      c->ctiles[i].fft_count = (int)c->fft_tiles.size() - c->ctiles[i].fft_first;
-    }
-    if (pass == 0) c->sb.n_fft_pow2 = (int)c->fft_tiles.size();
-  }
-  const int n_fft_pow2 = c->sb.n_fft_pow2;
-  int tri_lmax = 16;
+  // Poisson tile tables: capacity for the finest decomposition (one row per
+  // FFT tile, four modes per tridiagonal work-group)
   for (int i = 0; i < n_channels; ++i) {
-    const ChanDesc& d = c->desc[i];
-    c->ctiles[i].tri_first = (int)c->tri_tiles.size();
-    for (int k0 = 0; k0 < d.nk; k0 += d.tri_G) c->tri_tiles.push_back(make_int2(i, k0));
-    c->ctiles[i].tri_count = (int)c->tri_tiles.size() - c->ctiles[i].tri_first;
-    if (d.tri_L > 16) tri_lmax = 32;
+    c->fft_tiles_cap += (size_t)(c->desc[i].ny - 2);
+    c->tri_tiles_cap += (size_t)((c->desc[i].nk + 3) / 4);
   }
 
   // wall planes and twiddles
@@ -549,8 +597,8 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     }
     if ((e = dalloc(c, &c->d_desc, n_channels)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_lbm_tiles, c->lbm_tiles.size())) != hipSuccess) return e;
-    if ((e = dalloc(c, &c->d_fft_tiles, c->fft_tiles.size())) != hipSuccess) return e;
-    if ((e = dalloc(c, &c->d_tri_tiles, c->tri_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_fft_tiles, c->fft_tiles_cap)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_tri_tiles, c->tri_tiles_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_first_tile, c->first_tile.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_rho_e, c->total_nodes)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_psi, c->total_nodes)) != hipSuccess) return e;
@@ -578,8 +626,6 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     hipError_t e;
     if ((e = hipMemcpy(c->d_desc, c->desc.data(), n_channels * sizeof(ChanDesc), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_lbm_tiles, c->lbm_tiles.data(), c->lbm_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(c->d_fft_tiles, c->fft_tiles.data(), c->fft_tiles.size() * sizeof(int4), hipMemcpyHostToDevice)) != hipSuccess) return e;
-    if ((e = hipMemcpy(c->d_tri_tiles, c->tri_tiles.data(), c->tri_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_first_tile, c->first_tile.data(), c->first_tile.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_walls, c->walls.data(), c->walls.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return e;
@@ -604,17 +650,17 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     return EKPNP_ERR_HIP;
   }
   c->sb = make_buffers(c);
-  c->sb.n_fft_pow2 = n_fft_pow2;
-  // rows + quadrant twiddle table in LDS when the device allows that much
-  // dynamic LDS per work-group; otherwise twiddles are read from global memory
   int max_lds = 65536;
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device) != hipSuccess)
     max_lds = 65536;
   c->max_lds = max_lds;
-  c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)max_lds) ? 1 : 0;
-  c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
-  c->sb.fft_lds_dft = lds_dft;
-  c->sb.tri_lmax = tri_lmax;
+  he = build_poisson_plan(c, c->fft_elems, c->tri_rows);
+  if (he != hipSuccess) {
+    g_create_error = std::string("Poisson plan: ") + hipGetErrorString(he);
+    release(c);
+    delete c;
+    return EKPNP_ERR_HIP;
+  }
   for (int i = 0; i < n_channels; ++i) {
     if (upload_psi_walls(c, i) != hipSuccess) {
       g_create_error = "psi wall upload failed";
@@ -1016,6 +1062,70 @@ int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_s
     }
   }
   return thaw();
+}
+
+int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
+  if (!ctx || reps <= 0) return EKPNP_ERR_INVALID;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  // every trial recomputes psi from rho_e; the current psi is put back after
+  double* keep = nullptr;
+  HIP_OR_FAIL(ctx, hipMalloc((void**)&keep, ctx->total_nodes * sizeof(double)));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  auto finish = [&](int rc) {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipFree(keep);
+    return rc;
+  };
+  if (hipMemcpyAsync(keep, ctx->d_psi, ctx->total_nodes * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) !=
+          hipSuccess ||
+      hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess)
+    return finish(fail(ctx, EKPNP_ERR_HIP, "autotune set-up failed"));
+  const int elems[3] = {1024, 2048, 4096};
+  const int rows[2] = {8, 16};
+  int best_e = ctx->fft_elems, best_r = ctx->tri_rows;
+  double best = 1e30;
+  for (int ie = 0; ie < 3; ++ie) {
+    for (int ir = 0; ir < 2; ++ir) {
+      hipError_t e = build_poisson_plan(ctx, elems[ie], rows[ir]);
+      for (int w = 0; w < 2 && e == hipSuccess; ++w) e = enqueue_poisson(ctx, -1);
+      if (e == hipSuccess) e = hipEventRecord(ev[0], ctx->stream);
+      for (int k = 0; k < reps && e == hipSuccess; ++k) e = enqueue_poisson(ctx, -1);
+      if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (e != hipSuccess) return finish(fail(ctx, EKPNP_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e)));
+      const double per = (double)ms / reps;
+      if (per < best) {
+        best = per;
+        best_e = elems[ie];
+        best_r = rows[ir];
+      }
+    }
+  }
+  hipError_t e = build_poisson_plan(ctx, best_e, best_r);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->d_psi, keep, ctx->total_nodes * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return finish(fail(ctx, EKPNP_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e)));
+  if (out) {
+    out[0] = best_e;
+    out[1] = best_r;
+    out[2] = best;
+  }
+  return finish(EKPNP_OK);
+}
+
+int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (fft_elems < 1 || fft_elems > FFT_MAX_ELEMS || tri_rows < 1 || tri_rows > 16)
+    return fail(ctx, EKPNP_ERR_INVALID, "fft_elems must be in [1, 4096] and tri_rows in [1, 16]");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_OR_FAIL(ctx, build_poisson_plan(ctx, fft_elems, tri_rows));
+  return EKPNP_OK;
 }
 
 int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {

@@ -15,9 +15,9 @@ enum WallPlane { W_PSI_LO = 0, W_PSI_HI = 1, W_CP_LO = 2, W_CP_HI = 3, W_CN_LO =
 enum MacroPlane { M_RHO = 0, M_UX = 1, M_UY = 2, M_CP = 3, M_CN = 4, M_COUNT = 5 };
 
 constexpr int LBM_WG = 256;       // nodes per LBM work-group (one node per lane)
-constexpr int FFT_WG = 512;       // lanes of an x-FFT work-group
+constexpr int FFT_WG = 256;       // lanes of an x-FFT work-group
 constexpr int FFT_MAX_ELEMS = 4096;  // complex values held in LDS by one FFT work-group
-constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 8
+constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 16 values per lane per FFT pass
 constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT

@@ -478,90 +478,199 @@ __device__ __forceinline__ double2 tw_half(const double2* __restrict__ q, int j,
 }
 
 // ---------------------------------------------------------------------------
-// Stockham autosort FFT of `rows` rows of length N (a power of two) held in
-// LDS, in place, with a barrier between the reads and the writes of each
-// stage.  Stage (n, s): butterfly (p, q) reads x[q + s(p + j m)], writes
-// y[q + s(R p + j)], twiddle W_n^p (W_n = e^{SIGN 2 pi i/n}).  One radix-2
-// stage first when log2 N is odd, radix-4 after.  twq is the LDS quadrant
-// table of e^{-2 pi i j/nx}; W^2p and W^3p are formed by multiplication.
+// Register DFTs of size 2, 4, 8, 16 (natural order in and out), sign SIGN in
+// the exponent e^{SIGN 2 pi i jk/R}.  8 = 2 x 4 and 16 = 4 x 4 Cooley-Tukey
+// with the inner twiddles W_R^{j2 u1} as constants.
 // ---------------------------------------------------------------------------
+// v * (SIGN i)
 template <int SIGN>
-__device__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ twq,
-                         int lgnx) {
-  const int quarter = 1 << (lgnx - 2);
-  int lgn = lgN, lgs = 0;
-  if (lgn & 1) {
-    // radix 2
-    const int m = N >> 1;  // n = N at this stage
-    const int nb = rows * m, lgm = lgn - 1;
-    double2 va[FFT_PER_LANE / 2], vb[FFT_PER_LANE / 2];
+__device__ __forceinline__ double2 mul_si(double2 v) {
+  return SIGN < 0 ? make_double2(v.y, -v.x) : make_double2(-v.y, v.x);
+}
+// v * (c + SIGN i s)
+template <int SIGN>
+__device__ __forceinline__ double2 mul_cs(double2 v, double c, double sn) {
+  const double ss = SIGN < 0 ? -sn : sn;
+  return make_double2(v.x * c - v.y * ss, v.x * ss + v.y * c);
+}
+template <int SIGN>
+__device__ __forceinline__ void dft4(double2& a, double2& b, double2& c, double2& d) {
+  const double2 apc = cadd(a, c), amc = csub(a, c), bpd = cadd(b, d), jb = mul_si<SIGN>(csub(b, d));
+  a = cadd(apc, bpd);
+  b = cadd(amc, jb);
+  c = csub(apc, bpd);
+  d = csub(amc, jb);
+}
+
+constexpr double kC8 = 0.92387953251128675613;   // cos(pi/8)
+constexpr double kS8 = 0.38268343236508977173;   // sin(pi/8)
+constexpr double kH = 0.70710678118654752440;    // cos(pi/4)
+
+template <int SIGN, int R>
+__device__ __forceinline__ void dft_reg(double2* v) {
+  if (R == 2) {
+    const double2 a = v[0];
+    v[0] = cadd(a, v[1]);
+    v[1] = csub(a, v[1]);
+  } else if (R == 4) {
+    dft4<SIGN>(v[0], v[1], v[2], v[3]);
+  } else if (R == 8) {
+    // j = 4 j1 + j2, u = u1 + 2 u2
 #pragma unroll
-    for (int c = 0; c < FFT_PER_LANE / 2; ++c) {
-      const int bf = threadIdx.x + c * FFT_WG;
-      if (bf < nb) {
-        const int r = bf >> lgm, p = bf & (m - 1);
-        va[c] = x[(r << lgN) + p];
-        vb[c] = x[(r << lgN) + p + m];
-      }
+    for (int j2 = 0; j2 < 4; ++j2) {
+      const double2 a = v[j2];
+      v[j2] = cadd(a, v[4 + j2]);
+      v[4 + j2] = csub(a, v[4 + j2]);
     }
-    __syncthreads();
+    v[5] = mul_cs<SIGN>(v[5], kH, kH);
+    v[6] = mul_si<SIGN>(v[6]);
+    v[7] = mul_cs<SIGN>(v[7], -kH, kH);
+    dft4<SIGN>(v[0], v[1], v[2], v[3]);
+    dft4<SIGN>(v[4], v[5], v[6], v[7]);
+    // X[u1 + 2 u2] = v[4 u1 + u2]
+    const double2 t1 = v[1], t2 = v[2], t3 = v[3], t4 = v[4], t5 = v[5], t6 = v[6];
+    v[1] = t4;
+    v[2] = t1;
+    v[3] = t5;
+    v[4] = t2;
+    v[5] = t6;
+    v[6] = t3;
+  } else {
+    // R == 16: j = 4 j1 + j2, u = u1 + 4 u2
 #pragma unroll
-    for (int c = 0; c < FFT_PER_LANE / 2; ++c) {
-      const int bf = threadIdx.x + c * FFT_WG;
-      if (bf < nb) {
-        const int r = bf >> lgm, p = bf & (m - 1);
-        double2 w = tw_half(twq, p << (lgnx - lgn), quarter);
-        if (SIGN > 0) w = cconj(w);
-        x[(r << lgN) + 2 * p] = cadd(va[c], vb[c]);
-        x[(r << lgN) + 2 * p + 1] = cmul(csub(va[c], vb[c]), w);
-      }
-    }
-    __syncthreads();
-    lgn -= 1;
-    lgs = 1;
+    for (int j2 = 0; j2 < 4; ++j2) dft4<SIGN>(v[j2], v[4 + j2], v[8 + j2], v[12 + j2]);
+    // v[4 u1 + j2] *= W16^(u1 j2)
+    v[5] = mul_cs<SIGN>(v[5], kC8, kS8);    // 1
+    v[6] = mul_cs<SIGN>(v[6], kH, kH);      // 2
+    v[7] = mul_cs<SIGN>(v[7], kS8, kC8);    // 3
+    v[9] = mul_cs<SIGN>(v[9], kH, kH);      // 2
+    v[10] = mul_si<SIGN>(v[10]);            // 4
+    v[11] = mul_cs<SIGN>(v[11], -kH, kH);   // 6
+    v[13] = mul_cs<SIGN>(v[13], kS8, kC8);  // 3
+    v[14] = mul_cs<SIGN>(v[14], -kH, kH);   // 6
+    v[15] = mul_cs<SIGN>(v[15], -kC8, -kS8);  // 9
+#pragma unroll
+    for (int u1 = 0; u1 < 4; ++u1) dft4<SIGN>(v[4 * u1], v[4 * u1 + 1], v[4 * u1 + 2], v[4 * u1 + 3]);
+    // X[u1 + 4 u2] = v[4 u1 + u2]: a transpose of the 4 x 4 block
+    double2 t[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) t[i] = v[i];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] = t[4 * (u & 3) + (u >> 2)];
   }
-  while (lgn >= 2) {
-    const int lgm = lgn - 2, m = 1 << lgm, s = 1 << lgs;
-    const int lgper = lgN - 2;  // butterflies per row = N/4
-    const int nb = rows << lgper;
-    double2 v[FFT_PER_LANE];
+}
+
+// e^{SIGN 2 pi i t/nx} for 0 <= t < nx from the quadrant table
+template <int SIGN>
+__device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t, int lgnx) {
+  const int sh = lgnx - 2;
+  const int quad = t >> sh;
+  const double2 w = q[t & ((1 << sh) - 1)];
+  double2 r;
+  if (quad == 0) r = w;
+  else if (quad == 1) r = make_double2(w.y, -w.x);
+  else if (quad == 2) r = make_double2(-w.x, -w.y);
+  else r = make_double2(-w.y, w.x);
+  return SIGN > 0 ? cconj(r) : r;
+}
+
+// ---------------------------------------------------------------------------
+// Stockham autosort FFT of `rows` rows of length N = 2^lgN (4 <= N, and
+// rows * N <= 16 * FFT_WG) held in LDS, in place.  Stage (n, s) with radix R,
+// m = n / R: butterfly (p, q) reads x[q + s(p + j m)], j < R, takes the
+// R-point DFT in registers, multiplies output u by W_n^{p u} and writes
+// y[q + s(R p + u)]; a barrier separates the reads and the writes.  Radices:
+// one pass of 2^(lgN mod 4) when lgN is not a multiple of 4, then radix 16.
+// Each lane holds 16 values per pass.  twq is the quadrant table of e^{-2 pi i j/nx}.
+// ---------------------------------------------------------------------------
+template <int SIGN, int R>
+__device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int lgn, int lgs,
+                                              const double2* __restrict__ twq, int lgnx) {
+  constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
+  constexpr int BPT = 16 / R;
+  const int lgm = lgn - LGR, s = 1 << lgs;
+  const int lgper = lgN - LGR;
+  const int nb = rows << lgper;
+  double2 v[BPT][R];
 #pragma unroll
-    for (int c = 0; c < FFT_PER_LANE / 4; ++c) {
-      const int bf = threadIdx.x + c * FFT_WG;
-      if (bf < nb) {
-        const int r = bf >> lgper, j = bf & ((1 << lgper) - 1);
-        const int p = j >> lgs, qq = j & (s - 1);
-        const int base = (r << lgN) + qq + s * p;
+  for (int c = 0; c < BPT; ++c) {
+    const int bf = threadIdx.x + c * FFT_WG;
+    if (bf < nb) {
+      const int r = bf >> lgper, jj = bf & ((1 << lgper) - 1);
+      const int p = jj >> lgs, q = jj & (s - 1);
+      const int base = (r << lgN) + q + s * p;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[4 * c + u] = x[base + s * m * u];
+      for (int j = 0; j < R; ++j) v[c][j] = x[base + ((s * j) << lgm)];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < BPT; ++c) {
+    const int bf = threadIdx.x + c * FFT_WG;
+    if (bf < nb) {
+      const int r = bf >> lgper, jj = bf & ((1 << lgper) - 1);
+      const int p = jj >> lgs, q = jj & (s - 1);
+      dft_reg<SIGN, R>(v[c]);
+      const int ob = (r << lgN) + q + s * R * p;
+      x[ob] = v[c][0];
+      // W^(p u): table values for u = 1, 2, 4, 8, products (at most three
+      // roundings deep) for the rest
+      const int t1 = p << (lgnx - lgn);
+      const double2 w1 = tw_full<SIGN>(twq, t1, lgnx);
+      if (R == 2) {
+        x[ob + s] = cmul(v[c][1], w1);
+      } else {
+        const double2 w2 = tw_full<SIGN>(twq, 2 * t1, lgnx);
+        x[ob + s] = cmul(v[c][1], w1);
+        x[ob + 2 * s] = cmul(v[c][2], w2);
+        const double2 w3 = cmul(w2, w1);
+        x[ob + 3 * s] = cmul(v[c][3], w3);
+        if (R >= 8) {
+          const double2 w4 = tw_full<SIGN>(twq, 4 * t1, lgnx);
+          x[ob + 4 * s] = cmul(v[c][4], w4);
+          x[ob + 5 * s] = cmul(v[c][5], cmul(w4, w1));
+          x[ob + 6 * s] = cmul(v[c][6], cmul(w4, w2));
+          x[ob + 7 * s] = cmul(v[c][7], cmul(w4, w3));
+          if (R == 16) {
+            const double2 w8 = tw_full<SIGN>(twq, 8 * t1, lgnx);
+            x[ob + 8 * s] = cmul(v[c][8], w8);
+            x[ob + 9 * s] = cmul(v[c][9], cmul(w8, w1));
+            x[ob + 10 * s] = cmul(v[c][10], cmul(w8, w2));
+            x[ob + 11 * s] = cmul(v[c][11], cmul(w8, w3));
+            const double2 w12 = cmul(w8, w4);
+            x[ob + 12 * s] = cmul(v[c][12], w12);
+            x[ob + 13 * s] = cmul(v[c][13], cmul(w12, w1));
+            x[ob + 14 * s] = cmul(v[c][14], cmul(w12, w2));
+            x[ob + 15 * s] = cmul(v[c][15], cmul(w12, w3));
+          }
+        }
       }
     }
-    __syncthreads();
-#pragma unroll
-    for (int c = 0; c < FFT_PER_LANE / 4; ++c) {
-      const int bf = threadIdx.x + c * FFT_WG;
-      if (bf < nb) {
-        const int r = bf >> lgper, j = bf & ((1 << lgper) - 1);
-        const int p = j >> lgs, qq = j & (s - 1);
-        // W_n^p with p < n/4 lies in the first quadrant of the table
-        double2 w1 = twq[p << (lgnx - lgn)];
-        if (SIGN > 0) w1 = cconj(w1);
-        const double2 w2 = cmul(w1, w1), w3 = cmul(w2, w1);
-        const double2 a = v[4 * c + 0], bb = v[4 * c + 1], cc = v[4 * c + 2], d = v[4 * c + 3];
-        const double2 apc = cadd(a, cc), amc = csub(a, cc);
-        const double2 bpd = cadd(bb, d), bmd = csub(bb, d);
-        // forward: -i (b - d); inverse: +i (b - d)
-        const double2 jb = (SIGN < 0) ? make_double2(bmd.y, -bmd.x) : make_double2(-bmd.y, bmd.x);
-        const int ob = (r << lgN) + qq + s * 4 * p;
-        x[ob + 0 * s] = cadd(apc, bpd);
-        x[ob + 1 * s] = cmul(cadd(amc, jb), w1);
-        x[ob + 2 * s] = cmul(csub(apc, bpd), w2);
-        x[ob + 3 * s] = cmul(csub(amc, jb), w3);
-      }
-    }
-    __syncthreads();
+  }
+  __syncthreads();
+}
+
+template <int SIGN>
+__device__ __forceinline__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ twq, int lgnx) {
+  int lgn = lgN, lgs = 0;
+  const int r = lgN & 3;
+  if (r == 1) {
+    stockham_pass<SIGN, 2>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    lgn -= 1;
+    lgs += 1;
+  } else if (r == 2) {
+    stockham_pass<SIGN, 4>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 2;
     lgs += 2;
+  } else if (r == 3) {
+    stockham_pass<SIGN, 8>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    lgn -= 3;
+    lgs += 3;
+  }
+  while (lgn >= 4) {
+    stockham_pass<SIGN, 16>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    lgn -= 4;
+    lgs += 4;
   }
 }
 

@@ -197,6 +197,77 @@ def profiles(p: ChannelParams, y_norm: np.ndarray):
     return psi, cpos, cneg, ux
 
 
+# ---------------------------------------------------------------------------
+# physics baseline: Poisson-Boltzmann on the channel's own nodes plus the
+# Helmholtz-Smoluchowski velocity it implies
+# ---------------------------------------------------------------------------
+def pb_profile(p: ChannelParams, tol: float = 1e-13, max_iter: int = 200) -> np.ndarray:
+    """psi on nodes 0..ny-1 solving psi[i-1] - 2 psi[i] + psi[i+1] =
+    -dy^2 poisson_coef (c+ - c-), c+- = c_bulk exp(-+ psi inv_vt), with the
+    wall potentials as Dirichlet values (damped Newton, tridiagonal)."""
+    ny = p.ny
+    s2 = p.dy * p.dy * p.poisson_coef * p.c_bulk
+    ivt = p.inv_thermal_voltage
+    psi = np.linspace(p.psi_lo, p.psi_hi, ny)
+    psi[1:-1] *= 0.5
+    for _ in range(max_iter):
+        a = psi[1:-1]
+        f = psi[:-2] - 2.0 * a + psi[2:] - 2.0 * s2 * np.sinh(a * ivt)
+        diag = -2.0 - 2.0 * s2 * ivt * np.cosh(a * ivt)
+        step = _thomas(np.ones_like(a), diag, np.ones_like(a), -f)
+        lim = np.max(np.abs(step))
+        damp = 1.0 if lim <= 1.0 else 1.0 / lim
+        psi[1:-1] = a + damp * step
+        if lim < tol * max(1.0, np.max(np.abs(psi))):
+            break
+    return psi
+
+
+def _thomas(lo, diag, up, rhs):
+    n = diag.size
+    c = np.empty(n)
+    d = np.empty(n)
+    c[0] = up[0] / diag[0]
+    d[0] = rhs[0] / diag[0]
+    for i in range(1, n):
+        m = diag[i] - lo[i] * c[i - 1]
+        c[i] = up[i] / m
+        d[i] = (rhs[i] - lo[i] * d[i - 1]) / m
+    x = np.empty(n)
+    x[-1] = d[-1]
+    for i in range(n - 2, -1, -1):
+        x[i] = d[i] - c[i] * x[i + 1]
+    return x
+
+
+def baseline(p: ChannelParams):
+    """(psi, c+, c-, u_x) on the channel's ny rows: Poisson-Boltzmann potential,
+    Boltzmann ions, and u_x = hs_scale (psi - psi_w) / (-zeta) with the no-slip
+    plane half a node inside each wall (full-way bounce-back walls)."""
+    psi = pb_profile(p)
+    ivt = p.inv_thermal_voltage
+    cpos = p.c_bulk * np.exp(-psi * ivt)
+    cneg = p.c_bulk * np.exp(psi * ivt)
+    cpos[0], cpos[-1] = p.cpos_lo, p.cpos_hi
+    cneg[0], cneg[-1] = p.cneg_lo, p.cneg_hi
+    lo = 0.5 * (psi[0] + psi[1])
+    hi = 0.5 * (psi[-1] + psi[-2])
+    y = np.arange(p.ny, dtype=np.float64)
+    wall = lo + (hi - lo) * np.clip((y - 0.5) / max(p.ny - 2.0, 1.0), 0.0, 1.0)
+    ux = hs_scale(p) * (psi - wall) / (-p.psi_lo)
+    return psi, cpos, cneg, ux
+
+
+def residual_targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:
+    """targets() of a steady state minus targets() of the physics baseline."""
+    return targets(p, psi, cpos, cneg, ux) - targets(p, *baseline(p))
+
+
+def residual_profiles(p: ChannelParams, y_norm: np.ndarray):
+    """Inverse of residual_targets()."""
+    return profiles(p, y_norm + targets(p, *baseline(p)))
+
+
 def channel_of(zeta, kappa_h, e_x, nx, ny, tau=0.8, diff=0.05) -> ChannelParams:
     from . import lattice_params
     return lattice_params(int(nx), int(ny), tau=tau, zeta=float(zeta), kappa_h=float(kappa_h), e_x=float(e_x),

@@ -7,9 +7,13 @@
    u_x residual tolerance; their x-averaged steady profiles of psi, c+, c-,
    u_x are the targets.  The steady state of an x-uniform channel does not
    depend on nx, so training channels use a narrow nx.
-2. The MLP (4 x 256 GELU, BF16 MFMA kernels) is trained on the GPU.
-3. Held-out channels (another seed, nx from the config-2 set) are run twice:
-   cold (uniform c0 + noise) and warm (predicted profiles broadcast in x).
+2. The MLP (4 x 256 GELU, BF16 MFMA kernels) is trained on the GPU, by
+   default on the difference between those profiles and the physics
+   baseline (Poisson-Boltzmann on the channel's nodes + Helmholtz-
+   Smoluchowski u_x), so that BF16 only has to resolve a small correction.
+3. Held-out channels (another seed, nx from the config-2 set) are run three
+   times: cold (uniform c0 + noise), warm (predicted profiles broadcast in x)
+   and baseline-warm (the physics baseline alone).
    Every warm channel first runs a probe of `probe` steps; a channel whose
    residual after the probe exceeds `accept` (or is not finite) is rejected
    and restarted cold.  Both arms finish with the plain solver to the same
@@ -51,10 +55,11 @@ def run_cold(channels, precision, seed, tol, every, max_steps):
     return s, steps, time.perf_counter() - t0
 
 
-def generate(channels, precision, seed, tol, every, max_steps):
+def generate(channels, precision, seed, tol, every, max_steps, residual):
     s, steps, wall = run_cold(channels, precision, seed, tol, every, max_steps)
     X = S.channel_features(channels)
-    T = np.stack([S.targets(p, xavg(s, c, "psi"), xavg(s, c, "cpos"), xavg(s, c, "cneg"), xavg(s, c, "ux"))
+    tf = S.residual_targets if residual else S.targets
+    T = np.stack([tf(p, xavg(s, c, "psi"), xavg(s, c, "cpos"), xavg(s, c, "cneg"), xavg(s, c, "ux"))
                   for c, p in enumerate(channels)])
     s.close()
     return X, T, steps, wall
@@ -88,15 +93,26 @@ def train(X, T, epochs, batch, lr0, seed):
     return net, hist
 
 
-def run_warm(channels, net, precision, seed, tol, every, max_steps, probe, accept, corrupt=None):
+def predictor(net, residual):
+    """channels -> list of (psi, c+, c-, u_x) profiles; net None = physics baseline only."""
+    def run(channels):
+        if net is None:
+            return [S.baseline(p) for p in channels]
+        pred = net.forward(S.channel_features(channels))
+        inv = S.residual_profiles if residual else S.profiles
+        return [inv(p, pred[c]) for c, p in enumerate(channels)]
+    return run
+
+
+def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None):
     """Warm arm; corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
     t0 = time.perf_counter()
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
     cold = [{f: s.get_field(c, f) for f in ("rho", "ux", "uy", "cpos", "cneg")} for c in range(len(channels))]
-    pred = net.forward(S.channel_features(channels))
+    prof = predict(channels)
     for c, p in enumerate(channels):
-        psi, cp, cn, ux = S.profiles(p, pred[c])
+        psi, cp, cn, ux = prof[c]
         if corrupt is not None:
             psi, cp, cn, ux = corrupt(c, psi, cp, cn, ux)
         ny, nx = p.ny, p.nx
@@ -156,15 +172,21 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--save", default="")
+    ap.add_argument("--targets", choices=("direct", "residual"), default="residual",
+                    help="learn the profiles themselves or their difference from the physics baseline")
     args = ap.parse_args()
     ny = tuple(int(v) for v in args.ny.split(","))
     nxs = tuple(int(v) for v in args.nx.split(","))
 
     tr_ch, rej_tr = S.sample_channels(args.train, args.seed, nx_choices=(args.nx_train,), ny_choices=ny)
     log(f"training channels: {len(tr_ch)} (redrawn {rej_tr})")
-    X, T, tr_steps, tr_wall = generate(tr_ch, args.precision, args.seed, args.tol, args.every, args.max_steps)
+    residual = args.targets == "residual"
+    X, T, tr_steps, tr_wall = generate(tr_ch, args.precision, args.seed, args.tol, args.every, args.max_steps,
+                                       residual)
     good = tr_steps > 0
     log(f"data: {good.sum()} converged, steps median {np.median(tr_steps[good]):.0f}, wall {tr_wall:.1f} s")
+    rms = np.sqrt(np.mean(T[good].reshape(int(good.sum()), S.N_FIELDS, S.N_PTS) ** 2, axis=(0, 2)))
+    log(f"target rms per field (psi, c+, c-, u): {rms}")
     t0 = time.perf_counter()
     net, hist = train(X[good], T[good], args.epochs, args.batch, args.lr, args.seed)
     train_wall = time.perf_counter() - t0
@@ -173,10 +195,16 @@ def main():
     s, cold, cold_wall = run_cold(ho_ch, args.precision, 900000 + args.seed, args.tol, args.every, args.max_steps)
     s.close()
     log(f"cold: median steps {np.median(cold):.0f}, wall {cold_wall:.1f} s")
-    s, warm, warm_wall, probe_res, rejected = run_warm(ho_ch, net, args.precision, 900000 + args.seed, args.tol,
-                                                      args.every, args.max_steps, args.probe, args.accept)
+    s, warm, warm_wall, probe_res, rejected = run_warm(ho_ch, predictor(net, residual), args.precision,
+                                                      900000 + args.seed, args.tol, args.every, args.max_steps,
+                                                      args.probe, args.accept)
     s.close()
     log(f"warm: median steps {np.median(warm):.0f}, wall {warm_wall:.1f} s, rejected {len(rejected)}")
+    s, base, base_wall, base_res, base_rej = run_warm(ho_ch, predictor(None, residual), args.precision,
+                                                      900000 + args.seed, args.tol, args.every, args.max_steps,
+                                                      args.probe, args.accept)
+    s.close()
+    log(f"physics baseline: median steps {np.median(base):.0f}, wall {base_wall:.1f} s, rejected {len(base_rej)}")
     out = {
         "experiment": "time-to-steady, surrogate-warm vs cold",
         "tol": args.tol, "check_every": args.every, "precision": args.precision,
@@ -187,7 +215,11 @@ def main():
                     "wall_ratio": round(cold_wall / warm_wall, 3), "rejected": rejected,
                     "probe_residual_median": float(np.median(probe_res)),
                     "cold_steps": cold.tolist(), "warm_steps": warm.tolist(),
-                    "summary": summarize(cold, warm)},
+                    "summary": summarize(cold, warm),
+                    "baseline_steps": base.tolist(), "baseline_rejected": base_rej,
+                    "baseline_probe_residual_median": float(np.median(base_res)),
+                    "baseline_summary": summarize(cold, base)},
+        "targets": args.targets,
     }
     if args.save:
         np.savez(args.save, params=net.get_params(), X=X, T=T)

@@ -258,6 +258,19 @@ int ekpnp_record_umax(ekpnp_ctx* ctx, int channel, double time, const char* path
 int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path);
 int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
 
+/* ---- work decomposition ---------------------------------------------------- */
+/* Poisson plan: x-FFT work-groups hold up to fft_elems complex values (whole
+ * rows; default 2048), tridiagonal columns are cut into chunks of about
+ * tri_rows rows (default 16).  Results do not depend on fft_elems; tri_rows
+ * changes the FP64 rounding of psi only (chunk boundaries). */
+int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows);
+/* Times the Poisson stages over these channels for fft_elems in {1024,
+ * 2048, 4096} and tri_rows in {8, 16} (reps solves each, HIP events on the
+ * context's stream), keeps the fastest, and restores psi.  out (may be NULL)
+ * receives {fft_elems, tri_rows, ms per solve}.  No counterpart in the
+ * reference (cuFFT plans are fixed by main.cu:113). */
+int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
+
 /* ---- introspection / timing -------------------------------------------------- */
 /* The parameters channel was created with (wall profiles not included). */
 int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out);

@@ -113,9 +113,25 @@ def test_poisson_only(nx, ny):
     o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
     o.set_wall_profile(psi_lo=lo, psi_hi=hi)
     o.set_fields(np.ones(shp), np.zeros(shp), np.zeros(shp), cp, cn)
-    e = rel_l2(s.get_field(0, "psi"), o.get_field("psi"))
+    want = o.get_field("psi")
+    e = rel_l2(s.get_field(0, "psi"), want)
     print("poisson", nx, ny, e, flush=True)
     assert e < 1e-12
+    # every work decomposition solves the same system
+    for fft_elems, tri_rows in ((1024, 8), (4096, 16), (256, 4)):
+        s.set_poisson_plan(fft_elems, tri_rows)
+        s.fast_poisson()
+        e = rel_l2(s.get_field(0, "psi"), want)
+        print("poisson plan", fft_elems, tri_rows, e, flush=True)
+        assert e < 1e-12
+    # the autotuner keeps psi and leaves a valid plan behind
+    before = s.get_field(0, "psi")
+    info = s.autotune(3)
+    print("autotune", info, flush=True)
+    assert info["fft_elems"] in (1024, 2048, 4096) and info["tri_rows"] in (8, 16)
+    assert np.array_equal(s.get_field(0, "psi"), before)
+    s.fast_poisson()
+    assert rel_l2(s.get_field(0, "psi"), want) < 1e-12
 
 
 def test_batched_channels_match_single():

@@ -1,0 +1,71 @@
+"""Host-side pieces of the warm-start surrogate (no GPU): the physics
+baseline and the target transforms."""
+import numpy as np
+import pytest
+
+from ekpnp import surrogate as S
+
+
+def test_pb_profile_linear_limit_is_the_discrete_debye_huckel_solution():
+    # tiny zeta: sinh(psi) = psi to 1e-12, and the discrete equation
+    # psi[i-1] - (2 + kappa^2 dy^2) psi[i] + psi[i+1] = 0 is solved exactly by
+    # zeta cosh(theta (i - c)) / cosh(theta c), 2 cosh(theta) = 2 + kappa^2 dy^2
+    p = S.channel_of(-1e-6, 5.0, 1e-4, 4, 101)
+    psi = S.pb_profile(p)
+    k2 = 2.0 * p.poisson_coef * p.c_bulk * p.inv_thermal_voltage
+    th = np.arccosh(1.0 + 0.5 * k2 * p.dy ** 2)
+    i = np.arange(p.ny)
+    c = 0.5 * (p.ny - 1)
+    want = p.psi_lo * np.cosh(th * (i - c)) / np.cosh(th * c)
+    assert np.abs(psi - want).max() <= 1e-12 * abs(p.psi_lo)
+
+
+@pytest.mark.parametrize("zeta", [-1.0, -4.0])
+def test_pb_profile_thin_layer_matches_gouy_chapman(zeta):
+    # kappa h = 300 on 2001 nodes (kappa dy = 0.3): near each wall the channel
+    # is a semi-infinite Gouy-Chapman layer, tanh(psi/4) = tanh(zeta/4) e^{-kappa y};
+    # second-order discretisation error ~ (kappa dy)^2 / 12
+    p = S.channel_of(zeta, 300.0, 1e-4, 4, 2001)
+    psi = S.pb_profile(p)
+    kappa = np.sqrt(2.0 * p.poisson_coef)
+    y = np.arange(40) * p.dy
+    gc = 4.0 * np.arctanh(np.tanh(zeta / 4.0) * np.exp(-kappa * y))
+    err = np.abs(psi[:40] - gc).max() / abs(zeta)
+    assert err < 0.02
+    # discrete equation residual at the Newton solution
+    a = psi[1:-1]
+    f = psi[:-2] - 2 * a + psi[2:] - 2 * p.dy ** 2 * p.poisson_coef * np.sinh(a)
+    assert np.abs(f).max() < 1e-12
+
+
+def test_baseline_velocity_is_helmholtz_smoluchowski_in_the_bulk():
+    p = S.channel_of(-2.0, 40.0, 1e-4, 4, 256)
+    psi, cp, cn, ux = S.baseline(p)
+    mid = p.ny // 2
+    assert abs(psi[mid]) < 1e-6
+    # u = hs_scale (psi - psi_wall) / (-zeta) -> hs_scale (1 - ...) in the bulk
+    assert ux[mid] == pytest.approx(S.hs_scale(p) * (0.0 - 0.5 * (psi[0] + psi[1])) / 2.0, rel=1e-5)
+    assert np.allclose(cp[1:-1] * cn[1:-1], 1.0, rtol=1e-12)
+
+
+def test_residual_targets_round_trip():
+    p = S.channel_of(-1.5, 6.0, 3e-4, 4, 96)
+    rng = np.random.default_rng(0)
+    psi, cp, cn, ux = S.baseline(p)
+    psi2 = psi * (1 + 0.01 * rng.standard_normal(p.ny))
+    cp2 = cp * np.exp(0.01 * rng.standard_normal(p.ny))
+    cn2 = cn * np.exp(0.01 * rng.standard_normal(p.ny))
+    ux2 = ux + 1e-3 * S.hs_scale(p) * rng.standard_normal(p.ny)
+    psi2[0], psi2[-1] = p.psi_lo, p.psi_hi
+    cp2[0], cp2[-1] = p.cpos_lo, p.cpos_hi
+    cn2[0], cn2[-1] = p.cneg_lo, p.cneg_hi
+    t = S.residual_targets(p, psi2, cp2, cn2, ux2)
+    assert t.dtype == np.float32 and t.shape == (S.N_OUT,)
+    back = S.residual_profiles(p, t)
+    # 256 resampled points of a 96-node profile: linear resampling both ways
+    # reproduces the nodes to interpolation accuracy of the noise
+    for got, want in zip(back, (psi2, cp2, cn2, ux2)):
+        assert got.shape == (p.ny,)
+        assert np.abs(got - want).max() <= 0.05 * np.abs(want).max()
+    # the baseline itself maps to zero targets
+    assert np.abs(S.residual_targets(p, psi, cp, cn, ux)).max() < 1e-6
