@@ -25,6 +25,13 @@ namespace {
 
 const double W[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
+const int CXH[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+// storage references: one record of 8 doubles per row, X (rho, c+, c-),
+// J (their x-fluxes), 2 unused
+constexpr int kRefStride = 8;
+
+// stored value + stored_ref(y, d) = population (FP32 storage; 0 for FP64)
+std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err);
 
 thread_local std::string g_create_error;
 
@@ -72,7 +79,7 @@ struct ekpnp_ctx {
   double2* d_tw = nullptr;
   double2* d_spec = nullptr;
   double* d_ux_prev = nullptr;
-  double* d_refs = nullptr;      // storage references in use, [3][ny] per channel
+  double* d_refs = nullptr;      // storage references in use, [ny][8] per channel
   double* d_refs_new = nullptr;  // staging for a change of reference
   int2* d_row_tiles = nullptr;
   double* d_partials = nullptr;
@@ -89,7 +96,8 @@ struct ekpnp_ctx {
   long long snapshot_step = -1;
   long long rebase_step = 0;  // step count at the last change of storage reference
   // Poisson work decomposition (see build_poisson_plan)
-  int fft_elems = 2048, tri_rows = 16;
+  int fft_elems = 2048, tri_rows = 16, fft_persist = 1;
+  int n_cu = 256;
   size_t fft_tiles_cap = 0, tri_tiles_cap = 0;
   unsigned max_lds_needed = 0;
   std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
@@ -140,7 +148,7 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.active = 1;
   d.swap = 0;
   d.flags = (int)p.flags;
-  d.pow2 = (is_pow2(p.nx) && p.nx >= 4) ? 1 : 0;
+  d.pow2 = (is_pow2(p.nx) && p.nx >= 8) ? 1 : 0;
   const double cs2 = p.cs2, dt = p.dt;
   // relaxation rates, LBM.cu:336-341
   const double omega_plus = 1.0 / (p.nu / cs2 / dt + 1.0 / 2.0) / dt;
@@ -240,7 +248,7 @@ void drop_graphs(ekpnp_ctx* c) {
 // in the table.  Tridiagonal: each column is cut into C = 2^k chunks of at
 // most about tri_rows rows (8 <= C <= 128), G = TRI_WG / C modes per
 // work-group.  Rebuilding drops the cached graphs (their grids change).
-hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows) {
+hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft_persist) {
   c->fft_tiles.clear();
   c->tri_tiles.clear();
   unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;
@@ -300,8 +308,22 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows) {
   c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
   c->sb.fft_lds_dft = lds_dft;
   c->sb.tri_lmax = tri_lmax;
+  // the persistent kernels take tiles of <= FFT_PERSIST_ELEMS values and keep
+  // twiddles in LDS
+  bool persist_ok = fft_persist && fft_elems <= FFT_PERSIST_ELEMS && FFT_LDS_PERSIST <= (unsigned)c->max_lds;
+  for (const auto& d : c->desc) {
+    if (!d.pow2) continue;
+    const int N = d.nx / 2;
+    // rows, their spectra (N + 1 values each, FFT_PF + 1 per lane) and one
+    // X[N] slot per row must fit the persistent kernels' registers and LDS
+    if (N * d.fft_rows > FFT_PERSIST_ELEMS || (N + 1) * d.fft_rows > (FFT_PF + 1) * FFT_WG || d.fft_rows > 512)
+      persist_ok = false;
+  }
+  c->sb.fft_persist = persist_ok ? 1 : 0;
+  c->sb.fft_grid = std::max(1, std::min(n_fft_pow2, 2 * c->n_cu));
   c->fft_elems = fft_elems;
   c->tri_rows = tri_rows;
+  c->fft_persist = c->sb.fft_persist;
   drop_graphs(c);
   return hipSuccess;
 }
@@ -383,6 +405,20 @@ hipError_t sync_desc(ekpnp_ctx* c) {
 
 // lattice index holding channel ch's current state
 int chan_lattice(const ekpnp_ctx* c, int ch) { return c->cur ^ c->desc[ch].swap; }
+
+// [9][ny]: w_d (X(y) + c_dx J(y) / (cs^2 CFL)) of one species
+std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err) {
+  const ChanDesc& d = c->desc[channel];
+  std::vector<double> rec(kRefStride * (size_t)d.ny), out(9 * (size_t)d.ny);
+  *err = hipMemcpy(rec.data(), c->d_refs + d.ref_off, rec.size() * sizeof(double), hipMemcpyDeviceToHost);
+  const double tc = d.inv_cs2 * d.inv_cfl;
+  for (int dd = 0; dd < 9; ++dd)
+    for (int y = 0; y < d.ny; ++y) {
+      const double X = rec[kRefStride * (size_t)y + species], J = rec[kRefStride * (size_t)y + 3 + species];
+      out[dd * (size_t)d.ny + y] = W[dd] * (X + CXH[dd] * tc * J);
+    }
+  return out;
+}
 
 // psi wall rows of one channel from the wall planes
 hipError_t upload_psi_walls(ekpnp_ctx* c, int ch) {
@@ -541,7 +577,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     pop += 27 * d.plane;
     fld += pad64(nn);
     spec += pad64((long long)d.ny * d.nk_stride);
-    refs += pad64(3LL * d.ny);
+    refs += pad64((long long)kRefStride * d.ny);
     wall += (int)pad64(d.nx);
     c->desc[i] = d;
   }
@@ -654,7 +690,10 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, o.device) != hipSuccess)
     max_lds = 65536;
   c->max_lds = max_lds;
-  he = build_poisson_plan(c, c->fft_elems, c->tri_rows);
+  int n_cu = 256;
+  if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device) == hipSuccess && n_cu > 0)
+    c->n_cu = n_cu;
+  he = build_poisson_plan(c, c->fft_elems, c->tri_rows, c->fft_persist);
   if (he != hipSuccess) {
     g_create_error = std::string("Poisson plan: ") + hipGetErrorString(he);
     release(c);
@@ -926,9 +965,9 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   const ChanDesc& d = ctx->desc[channel];
   const size_t nn = (size_t)d.nx * d.ny;
-  std::vector<double> ref(d.ny);
-  HIP_OR_FAIL(ctx, hipMemcpy(ref.data(), ctx->d_refs + d.ref_off + (size_t)species * d.ny, d.ny * sizeof(double),
-                             hipMemcpyDeviceToHost));
+  hipError_t re = hipSuccess;
+  const std::vector<double> ref = ref_offsets(ctx, channel, species, &re);
+  HIP_OR_FAIL(ctx, re);
   const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
   const char* base = (const char*)src + off * ctx->elem;
   if (ctx->elem == 8) {
@@ -939,7 +978,7 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
     for (size_t i = 0; i < 9 * nn; ++i) host_out[i] = (double)tmp[i];
   }
   for (int dd = 0; dd < 9; ++dd)
-    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += W[dd] * ref[i / d.nx];
+    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += ref[dd * d.ny + i / d.nx];
   return EKPNP_OK;
 }
 
@@ -959,18 +998,18 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double
   const size_t off = (size_t)d.pop_off + (size_t)(9 * species) * (size_t)d.plane;
   char* base = (char*)ctx->lat[chan_lattice(ctx, channel)] + off * ctx->elem;
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  std::vector<double> ref(d.ny);
-  HIP_OR_FAIL(ctx, hipMemcpy(ref.data(), ctx->d_refs + d.ref_off + (size_t)species * d.ny, d.ny * sizeof(double),
-                             hipMemcpyDeviceToHost));
+  hipError_t re = hipSuccess;
+  const std::vector<double> ref = ref_offsets(ctx, channel, species, &re);
+  HIP_OR_FAIL(ctx, re);
   if (ctx->elem == 8) {
     std::vector<double> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - W[dd] * ref[i / d.nx];
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - ref[dd * d.ny + i / d.nx];
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 8, tmp.data(), nn * 8, nn * 8, 9, hipMemcpyHostToDevice));
   } else {
     std::vector<float> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - W[dd] * ref[i / d.nx]);
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - ref[dd * d.ny + i / d.nx]);
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 4, tmp.data(), nn * 4, nn * 4, 9, hipMemcpyHostToDevice));
   }
   ctx->has_pops = true;
@@ -1084,11 +1123,13 @@ int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
     return finish(fail(ctx, EKPNP_ERR_HIP, "autotune set-up failed"));
   const int elems[3] = {1024, 2048, 4096};
   const int rows[2] = {8, 16};
-  int best_e = ctx->fft_elems, best_r = ctx->tri_rows;
+  int best_e = ctx->fft_elems, best_r = ctx->tri_rows, best_p = ctx->fft_persist;
   double best = 1e30;
-  for (int ie = 0; ie < 3; ++ie) {
+  for (int ic = 0; ic < 6; ++ic) {
+    const int ie = ic % 3, ip = ic / 3;  // classic kernels for every size, persistent up to 2048
+    if (ip == 1 && elems[ie] > FFT_PERSIST_ELEMS) continue;
     for (int ir = 0; ir < 2; ++ir) {
-      hipError_t e = build_poisson_plan(ctx, elems[ie], rows[ir]);
+      hipError_t e = build_poisson_plan(ctx, elems[ie], rows[ir], ip);
       for (int w = 0; w < 2 && e == hipSuccess; ++w) e = enqueue_poisson(ctx, -1);
       if (e == hipSuccess) e = hipEventRecord(ev[0], ctx->stream);
       for (int k = 0; k < reps && e == hipSuccess; ++k) e = enqueue_poisson(ctx, -1);
@@ -1102,10 +1143,11 @@ int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
         best = per;
         best_e = elems[ie];
         best_r = rows[ir];
+        best_p = ctx->fft_persist;
       }
     }
   }
-  hipError_t e = build_poisson_plan(ctx, best_e, best_r);
+  hipError_t e = build_poisson_plan(ctx, best_e, best_r, best_p);
   if (e == hipSuccess)
     e = hipMemcpyAsync(ctx->d_psi, keep, ctx->total_nodes * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1114,17 +1156,18 @@ int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
     out[0] = best_e;
     out[1] = best_r;
     out[2] = best;
+    out[3] = ctx->fft_persist;
   }
   return finish(EKPNP_OK);
 }
 
-int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows) {
+int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent) {
   if (!ctx) return EKPNP_ERR_INVALID;
   if (fft_elems < 1 || fft_elems > FFT_MAX_ELEMS || tri_rows < 1 || tri_rows > 16)
     return fail(ctx, EKPNP_ERR_INVALID, "fft_elems must be in [1, 4096] and tri_rows in [1, 16]");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_OR_FAIL(ctx, build_poisson_plan(ctx, fft_elems, tri_rows));
+  HIP_OR_FAIL(ctx, build_poisson_plan(ctx, fft_elems, tri_rows, fft_persistent ? 1 : 0));
   return EKPNP_OK;
 }
 
@@ -1142,7 +1185,7 @@ int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {
 //   int64 steps, snapshot_step, rebase_step;
 //   per channel: int32 nx, ny;
 //   per channel: populations [27][ny][nx] in the storage type, references
-//   [3][ny], psi, rho, ux, uy, c+, c- [ny][nx], wall planes [6][nx],
+//   [ny][8], psi, rho, ux, uy, c+, c- [ny][nx], wall planes [6][nx],
 //   residual snapshot [ny][nx] (FP64 unless stated).
 // ---------------------------------------------------------------------------
 namespace {
@@ -1183,7 +1226,7 @@ int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
     const char* lat = (const char*)ctx->lat[chan_lattice(ctx, ch)] + (size_t)d.pop_off * e;
     HIP_OR_FAIL(ctx, hipMemcpy2D(buf.data(), nn * e, lat, (size_t)d.plane * e, nn * e, 27, hipMemcpyDeviceToHost));
     put(buf.data(), 27 * nn * e);
-    std::vector<double> refs(3 * (size_t)d.ny);
+    std::vector<double> refs(kRefStride * (size_t)d.ny);
     HIP_OR_FAIL(ctx, hipMemcpy(refs.data(), ctx->d_refs + d.ref_off, refs.size() * sizeof(double),
                                hipMemcpyDeviceToHost));
     put(refs.data(), refs.size() * sizeof(double));
@@ -1235,7 +1278,7 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
     if (!get(buf.data(), 27 * nn * e)) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
     char* lat = (char*)ctx->lat[ctx->cur] + (size_t)d.pop_off * e;
     HIP_OR_FAIL(ctx, hipMemcpy2D(lat, (size_t)d.plane * e, buf.data(), nn * e, nn * e, 27, hipMemcpyHostToDevice));
-    std::vector<double> refs(3 * (size_t)d.ny);
+    std::vector<double> refs(kRefStride * (size_t)d.ny);
     if (!get(refs.data(), refs.size() * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
     HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_refs + d.ref_off, refs.data(), refs.size() * sizeof(double),
                                hipMemcpyHostToDevice));

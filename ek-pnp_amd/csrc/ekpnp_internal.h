@@ -18,6 +18,11 @@ constexpr int LBM_WG = 256;       // nodes per LBM work-group (one node per lane
 constexpr int FFT_WG = 256;       // lanes of an x-FFT work-group
 constexpr int FFT_MAX_ELEMS = 4096;  // complex values held in LDS by one FFT work-group
 constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 16 values per lane per FFT pass
+// persistent pipelined x-FFT: tiles of <= FFT_PERSIST_ELEMS complex values,
+// 8 prefetched per lane; LDS = rows + quadrant twiddles + one X[N] per row
+constexpr int FFT_PF = 8;
+constexpr int FFT_PERSIST_ELEMS = FFT_PF * FFT_WG;  // 2048
+constexpr unsigned FFT_LDS_PERSIST = (FFT_PERSIST_ELEMS + FFT_PERSIST_ELEMS / 2 + 512) * 16;
 constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
@@ -33,7 +38,7 @@ struct ChanDesc {
   long long plane;              // population plane stride (nx*ny rounded up to 64)
   long long fld_off;            // first element of this channel's [ny][nx] fields
   long long spec_off;           // first double2 of this channel's spectrum blocks
-  long long ref_off;            // first double of this channel's [3][ny] storage references
+  long long ref_off;            // first double of this channel's [ny][8] storage references
   int wall_off;                 // offset into each wall plane
   int tw_off;                   // offset of this channel's twiddles e^{-2 pi i j/nx}
   int pow2;                     // nx is a power of two (Stockham path)
@@ -63,6 +68,8 @@ struct StepBuffers {
   const int4* fft_tiles;        // (channel, first row, rows, 0) per FFT work-group
   int n_fft_tiles;
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
+  int fft_persist;              // power-of-two tiles by the persistent kernels
+  int fft_grid;                 // work-groups of the persistent kernels
   unsigned fft_lds_pow2;        // dynamic LDS bytes of the Stockham kernels
   unsigned fft_lds_dft;         // dynamic LDS bytes of the direct-DFT kernels
   int fft_tw_lds;               // the power-of-two kernels stage twiddles in LDS
@@ -82,7 +89,7 @@ struct StepBuffers {
   double* red_out;              // reductions [n_channels][8]
   const int* red_first_tile;    // first LBM tile of each channel (n_channels + 1)
   int n_channels;
-  double* refs;                 // storage references, [3][ny] per channel (rho, c+, c-)
+  double* refs;                 // storage references, [ny][8] per channel (rho, c+, c-, their x-fluxes, 2 unused)
   const int2* row_tiles;        // (channel, row) per row-means work-group
   int n_row_tiles;
 };
@@ -103,7 +110,7 @@ hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s);
 // PB-Picard start: Boltzmann concentrations from psi, then under-relaxation
 // of psi against psi_old (psi_old is updated to the relaxed value)
 hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s);
-// row means over x of the macroscopic rho, c+, c- into out ([3][ny] per channel)
+// row means over x of rho, c+, c- and rho u_x, c+ u_x, c- u_x into out ([ny][8] per channel)
 hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s);
 // re-express every stored population against new_refs (state in cur, or in
 // other for channels whose swap flag is set)

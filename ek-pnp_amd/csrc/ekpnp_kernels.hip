@@ -98,29 +98,61 @@ __device__ __forceinline__ void efield_at(const double* __restrict__ ps, int nx,
   Ey = (ps[(long long)ya * nx + x] - ps[(long long)yb * nx + x]) * inv2dy;
 }
 
-// Storage reference of a row: the populations stored at a node of row y are
-// deviations from w_i * (rho, c+, c-)_ref(y) (all zero for FP64 storage).
-// R points at the channel's [3][ny] reference rows.
-struct RowRef {
-  double r, p, n;
+// Storage reference of a row (FP32 storage; all zero for FP64 storage).  R
+// points at the channel's table of one 64-byte record per row:
+// X_s(y) = (rho, c+, c-) and J_s(y) = the x-fluxes (rho u_x, c+ u_x, c- u_x),
+// row means, then two unused slots.  A stored population is
+//   stored_i = f_i - w_i X_s(y) - w_i c_ix tc J_s(y),   tc = 1/(cs^2 CFL),
+// i.e. the departure from the first-order equilibrium of the row means.  The
+// collision works on the rest deviation q_i = f_i - w_i X_s(y).
+struct RowRefs {
+  double X[3], J[3];
 };
-__device__ __forceinline__ RowRef row_ref(const double* __restrict__ R, int ny, int y) {
-  return RowRef{R[y], R[ny + y], R[2 * ny + y]};
+constexpr int REF_ROWS = 6;     // values per row record
+constexpr int REF_STRIDE = 8;   // doubles per row record
+__device__ __forceinline__ RowRefs load_refs(const double* __restrict__ R, int y) {
+  const double2* p = reinterpret_cast<const double2*>(R + (long long)REF_STRIDE * y);
+  const double2 a = p[0], b = p[1], c = p[2];
+  RowRefs r;
+  r.X[0] = a.x;
+  r.X[1] = a.y;
+  r.X[2] = b.x;
+  r.J[0] = b.y;
+  r.J[1] = c.x;
+  r.J[2] = c.y;
+  return r;
+}
+// the row records a node of row y reads: rows y-1, y, y+1 (periodic in y)
+struct Refs3 {
+  RowRefs m, o, p;
+};
+__device__ __forceinline__ Refs3 load_refs3(const double* __restrict__ R, int ny, int y) {
+  const int ym = (y == 0) ? ny - 1 : y - 1;
+  const int yp = (y == ny - 1) ? 0 : y + 1;
+  return Refs3{load_refs(R, ym), load_refs(R, y), load_refs(R, yp)};
 }
 
-// Pre-collision populations of node (x, y), 27 values: f 0..8, h 9..17,
-// g 18..26, as deviations from row y's reference.  FROM_Q: the lattice holds
-// them unshifted.  Otherwise gather the post-collision values of the upstream
-// neighbours (pull streaming); values streamed from rows y-1 and y+1 are
-// re-expressed against row y's reference.
+// direction x-components c_ix of D2Q9 (0, E, N, W, S, NE, NW, SW, SE)
+__device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
+
+// rest deviation of node (x, y) from stored values at their own rows
+// (FROM_Q) or gathered from the upstream neighbours (pull streaming):
+//   q_i = stored_i(src) + w_i (X(src) - X(y)) + w_i c_ix tc J(src).
+// 27 values: f 0..8, h 9..17, g 18..26.
 template <typename S, typename C, bool FROM_Q>
 __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plane, int nx, int ny,
-                                         int x, int y, int xm, int xp, const double* __restrict__ R,
-                                         C* q) {
+                                         int x, int y, int xm, int xp, const Refs3& rf, double tc, C* q) {
   const long long r0 = (long long)y * nx;
   if (FROM_Q) {
 #pragma unroll
-    for (int k = 0; k < 27; ++k) q[k] = (C)ld_pop(P + k * plane + r0 + x);
+    for (int s = 0; s < 3; ++s) {
+      const C j = C(rf.o.J[s] * tc);
+#pragma unroll
+      for (int d = 0; d < 9; ++d) {
+        q[9 * s + d] = (C)ld_pop(P + (9 * s + d) * plane + r0 + x);
+        if (CX[d] != 0) q[9 * s + d] += C(CX[d] * (d < 5 ? WS : WD)) * j;
+      }
+    }
     return;
   }
   const int ym = (y == 0) ? ny - 1 : y - 1;
@@ -129,8 +161,9 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const S* B = P + (long long)(9 * s) * plane;
-    const double own = R[s * ny + y];
-    const C dm = C(R[s * ny + ym] - own), dp = C(R[s * ny + yp] - own);
+    const double own = rf.o.X[s];
+    const C dm = C(rf.m.X[s] - own), dp = C(rf.p.X[s] - own);
+    const C j0 = C(rf.o.J[s] * tc), jm = C(rf.m.J[s] * tc), jp = C(rf.p.J[s] * tc);
     q[9 * s + 0] = (C)ld_pop(B + 0 * plane + r0 + x);
     q[9 * s + 1] = (C)ld_pop(B + 1 * plane + r0 + xm);
     q[9 * s + 2] = (C)ld_pop(B + 2 * plane + rm + x);
@@ -140,31 +173,41 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
     q[9 * s + 6] = (C)ld_pop(B + 6 * plane + rm + xp);
     q[9 * s + 7] = (C)ld_pop(B + 7 * plane + rp + xp);
     q[9 * s + 8] = (C)ld_pop(B + 8 * plane + rp + xm);
+    q[9 * s + 1] += C(WS) * j0;
+    q[9 * s + 3] -= C(WS) * j0;
     q[9 * s + 2] += C(WS) * dm;
-    q[9 * s + 5] += C(WD) * dm;
-    q[9 * s + 6] += C(WD) * dm;
+    q[9 * s + 5] += C(WD) * (dm + jm);
+    q[9 * s + 6] += C(WD) * (dm - jm);
     q[9 * s + 4] += C(WS) * dp;
-    q[9 * s + 7] += C(WD) * dp;
-    q[9 * s + 8] += C(WD) * dp;
+    q[9 * s + 7] += C(WD) * (dp - jp);
+    q[9 * s + 8] += C(WD) * (dp + jp);
   }
+}
+
+// stored value of rest deviation v in direction d of a row with flux term j = tc J
+template <typename C>
+__device__ __forceinline__ C to_stored(C v, int d, C j) {
+  return CX[d] == 0 ? v : v - C(CX[d] * (d < 5 ? WS : WD)) * j;
 }
 
 // Ion wall node: the next pre-collision state is the anti-bounce-back of the
 // node's own post-collision state, q[opp(i)] = -p[i] + 2 w_i c_w
-// (LBM.cu:935-943), written here for storage deviations (dcw = c_w - c_ref).
+// (LBM.cu:935-943), written here for rest deviations (dcw = c_w - c_ref; the
+// stored p[i] gets its flux term j = tc J back first).
 template <typename S, typename C>
 __device__ __forceinline__ void abb_own(const S* __restrict__ P, long long plane, long long node,
-                                        C dcw, C* q) {
+                                        C dcw, C j, C* q) {
   const C two_s = C(2.0 * WS) * dcw, two_d = C(2.0 * WD) * dcw;
+  const C js = C(WS) * j, jd = C(WD) * j;
   q[0] = -(C)P[0 * plane + node] + C(2.0 * W0) * dcw;
-  q[3] = -(C)P[1 * plane + node] + two_s;
+  q[3] = -((C)P[1 * plane + node] + js) + two_s;
   q[4] = -(C)P[2 * plane + node] + two_s;
-  q[1] = -(C)P[3 * plane + node] + two_s;
+  q[1] = -((C)P[3 * plane + node] - js) + two_s;
   q[2] = -(C)P[4 * plane + node] + two_s;
-  q[7] = -(C)P[5 * plane + node] + two_d;
-  q[8] = -(C)P[6 * plane + node] + two_d;
-  q[5] = -(C)P[7 * plane + node] + two_d;
-  q[6] = -(C)P[8 * plane + node] + two_d;
+  q[7] = -((C)P[5 * plane + node] + jd) + two_d;
+  q[8] = -((C)P[6 * plane + node] - jd) + two_d;
+  q[5] = -((C)P[7 * plane + node] - jd) + two_d;
+  q[6] = -((C)P[8 * plane + node] + jd) + two_d;
 }
 
 // ---------------------------------------------------------------------------
@@ -241,14 +284,14 @@ struct Moments {
 };
 
 template <typename C>
-__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, const RowRef& rr,
+__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, const RowRefs& rr,
                                               double Ex, double Ey) {
   Moments<C> m;
   m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-  m.rho = C(rr.r) + m.dr;
+  m.rho = C(rr.X[0]) + m.dr;
   m.dcp = ((q[9] + q[10]) + (q[11] + q[12])) + ((q[13] + q[14]) + (q[15] + q[16])) + q[17];
   m.dcn = ((q[18] + q[19]) + (q[20] + q[21])) + ((q[22] + q[23]) + (q[24] + q[25])) + q[26];
-  m.rhoe = (m.dcp - m.dcn) + C(rr.p - rr.n);
+  m.rhoe = (m.dcp - m.dcn) + C(rr.X[1] - rr.X[2]);
   const C jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
   const C jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
   m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
@@ -285,30 +328,37 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
   const bool bottom = (y == 0), top = (y == ny - 1);
-  const RowRef rr = row_ref(R, ny, y);
+  const double tc = cd->inv_cs2 * cd->inv_cfl;
+  // A work-group inside one row (every tile when nx is a multiple of LBM_WG)
+  // reads its three row records with scalar loads.
+  Refs3 rf;
+  if (tile.y % nx + LBM_WG <= nx) rf = load_refs3(R, ny, tile.y / nx);
+  else rf = load_refs3(R, ny, y);
+  const RowRefs& rr = rf.o;
 
   // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
   // included (LBM.cu:402-443); evaluated first so its registers are free.
   C mux = 0, muy = 0;
   if (bottom) {
     C m1[27];
-    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, R, m1);
+    const Refs3 r1 = load_refs3(R, ny, 1);
+    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, r1, tc, m1);
     double Ex1, Ey1;
     efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
-    const Moments<C> mm = moments(m1, cd, row_ref(R, ny, 1), Ex1, Ey1);
+    const Moments<C> mm = moments(m1, cd, r1.o, Ex1, Ey1);
     mux = -mm.ux;
     muy = -mm.uy;
   }
 
   C q[27];
-  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, R, q);
+  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
   if (!FROM_Q && (bottom || top)) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.p), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.n), q + 18);
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.X[1]), C(rr.J[1] * tc), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.X[2]), C(rr.J[2] * tc), q + 18);
   }
 
   double Ex, Ey;
@@ -328,22 +378,22 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
       sp += (double)q[9 + k];
       sn += (double)q[18 + k];
     }
-    b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.p - rr.n);
+    b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.X[1] - rr.X[2]);
   }
   if (MACRO) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
     b.macro[M_RHO * st + o] = (double)m.rho;
     b.macro[M_UX * st + o] = (double)ux;
     b.macro[M_UY * st + o] = (double)uy;
-    b.macro[M_CP * st + o] = (double)(C(rr.p) + m.dcp);
-    b.macro[M_CN * st + o] = (double)(C(rr.n) + m.dcn);
+    b.macro[M_CP * st + o] = (double)(C(rr.X[1]) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(rr.X[2]) + m.dcn);
   }
 
   const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
   // ions: drift velocity u + K E (LBM.cu:479-488)
   {
     const C Eix = C(Ex + cd->drift_app), Eiy = C(Ey);
-    const C cp = C(rr.p) + m.dcp, cn = C(rr.n) + m.dcn;
+    const C cp = C(rr.X[1]) + m.dcp, cn = C(rr.X[2]) + m.dcn;
     const C kp = C(cd->kp), kn = C(cd->kn);
     const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
     const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
@@ -373,7 +423,11 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
                   C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
   }
 #pragma unroll
-  for (int k = 0; k < 27; ++k) st_pop(Q + k * plane + node, (S)q[k]);
+  for (int s = 0; s < 3; ++s) {
+    const C j = C(rr.J[s] * tc);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st_pop(Q + (9 * s + d) * plane + node, (S)to_stored(q[9 * s + d], d, j));
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -402,21 +456,21 @@ __global__ __launch_bounds__(LBM_WG) void k_init_eq(StepBuffers b, S* __restrict
   const double vel[3][2] = {{ux, uy}, {ux + cd->kp * Ex, uy + cd->kp * Ey},
                             {ux + cd->kn * Ex, uy + cd->kn * Ey}};
   const double X[3] = {rho, cp, cn};
-  const RowRef rr = row_ref(b.refs + cd->ref_off, ny, y);
-  const double ref[3] = {rr.r, rr.p, rr.n};
+  const RowRefs rr = load_refs(b.refs + cd->ref_off, y);
   S* __restrict__ Q = dst + cd->pop_off;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
+    const double j = rr.J[s] * ticf;
     const double vx = vel[s][0], vy = vel[s][1];
     const double om = -0.5 * (vx * vx + vy * vy) * ics;
     const double tx = vx * ticf, ty = vy * ticf;
-    const double dX = X[s] - ref[s];
+    const double dX = X[s] - rr.X[s];
     const double t[9] = {0.0, tx, ty, -tx, -ty, tx + ty, ty - tx, -(tx + ty), tx - ty};
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
       const double w = (d == 0) ? W0 : (d < 5 ? WS : WD);
       const double eq = w * (dX + X[s] * (om + t[d] * (1.0 + 0.5 * t[d])));
-      Q[(9 * s + d) * plane + node] = (S)eq;
+      Q[(9 * s + d) * plane + node] = (S)to_stored(eq, d, j);
     }
   }
 }
@@ -438,20 +492,26 @@ __global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restr
   const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
   const double* __restrict__ R = b.refs + cd->ref_off;
-  const RowRef rr = row_ref(R, ny, y);
+  const Refs3 rf = load_refs3(R, ny, y);
+  const RowRefs& rr = rf.o;
+  const double tc = cd->inv_cs2 * cd->inv_cfl;
   C q[27];
-  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, R, q);
+  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
   if (y == 0 || y == ny - 1) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = (y == 0) ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = (y == 0) ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.p), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.n), q + 18);
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.X[1]), C(rr.J[1] * tc), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.X[2]), C(rr.J[2] * tc), q + 18);
   }
   S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
 #pragma unroll
-  for (int k = 0; k < 27; ++k) Q[k * plane + node] = (S)q[k];
+  for (int s = 0; s < 3; ++s) {
+    const C j = C(rr.J[s] * tc);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) Q[(9 * s + d) * plane + node] = (S)to_stored(q[9 * s + d], d, j);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -766,6 +826,203 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
 }
 
 // ---------------------------------------------------------------------------
+// Persistent, software-pipelined power-of-two x-FFTs.  A work-group walks the
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ...; while it transforms tile i
+// in LDS, the loads of tile i+1 are in flight into registers (FFT_PF values
+// per lane).  Tiles hold rows * N <= FFT_PF * FFT_WG complex values.  The
+// quadrant twiddle table is (re)loaded into LDS when the channel changes.
+// ---------------------------------------------------------------------------
+static_assert(FFT_PF * FFT_WG == FFT_PERSIST_ELEMS, "persistent FFT tile size");
+
+struct FwdPrefetch {
+  double2 v[FFT_PF];
+};
+
+__device__ __forceinline__ void fwd_prefetch(const StepBuffers& b, int t, FwdPrefetch& pf) {
+  const int4 tt = b.fft_tiles[t];
+  const ChanDesc* __restrict__ cd = b.chans + tt.x;
+  const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), rows = tt.z;
+  const double2* __restrict__ r2 = reinterpret_cast<const double2*>(b.rho_e + cd->fld_off);
+  const int last = rows * N - 1;
+#pragma unroll
+  for (int c = 0; c < FFT_PF; ++c) {
+    const int e = min((int)threadIdx.x + c * FFT_WG, last);
+    const int r = e >> lgN, n = e & (N - 1);
+    pf.v[c] = r2[(((long long)(tt.y + r) * nx) >> 1) + n];
+  }
+}
+
+__global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, int n_tiles) {
+  extern __shared__ double2 lds[];
+  const int max_elems = FFT_PF * FFT_WG;
+  double2* twl = lds + max_elems;  // quadrant twiddle table after the rows
+  int tw_chan = -1;
+  FwdPrefetch pf;
+  if ((int)blockIdx.x < n_tiles) fwd_prefetch(b, blockIdx.x, pf);
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int4 tt = b.fft_tiles[t];
+    const ChanDesc* __restrict__ cd = b.chans + tt.x;
+    const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
+    const int y0 = tt.y, rows = tt.z;
+    const bool active = cd->active != 0;
+    if (active) {
+      if (tt.x != tw_chan) {
+        const double2* __restrict__ tw = b.tw + cd->tw_off;
+        for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
+        tw_chan = tt.x;
+      }
+      const double* __restrict__ Wl = b.walls + cd->wall_off;
+      const long long ws = b.walls_stride;
+#pragma unroll
+      for (int c = 0; c < FFT_PF; ++c) {
+        const int e = threadIdx.x + c * FFT_WG;
+        if (e < rows * N) {
+          const int r = e >> lgN, n = e & (N - 1), y = y0 + r;
+          double v0 = -cd->pc_dy2 * pf.v[c].x, v1 = -cd->pc_dy2 * pf.v[c].y;
+          if (y == 1) {
+            v0 -= Wl[W_PSI_LO * ws + 2 * n];
+            v1 -= Wl[W_PSI_LO * ws + 2 * n + 1];
+          }
+          if (y == cd->ny - 2) {
+            v0 -= Wl[W_PSI_HI * ws + 2 * n];
+            v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
+          }
+          lds[e] = make_double2(v0, v1);
+        }
+      }
+    }
+    // the next tile's loads overlap this tile's transform
+    if (t + (int)gridDim.x < n_tiles) fwd_prefetch(b, t + gridDim.x, pf);
+    if (active) {
+      __syncthreads();
+      stockham<-1>(lds, rows, N, lgN, twl, lgnx);
+      double2* __restrict__ spec = b.spec + cd->spec_off;
+      for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
+        const int k = e / rows, r = e - k * rows;
+        const double2 zk = lds[(r << lgN) + (k & (N - 1))];
+        const double2 zc = cconj(lds[(r << lgN) + ((N - k) & (N - 1))]);
+        const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+        const double2 dd = csub(zk, zc);
+        const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
+        spec[spec_idx(cd->ny, y0 + r, k)] = cadd(E, cmul(tw_half(twl, k, quarter), O));
+      }
+      __syncthreads();  // LDS rows and twiddles are rewritten next
+    }
+  }
+}
+
+constexpr int FFT_PF_INV = FFT_PF + 1;  // rows * (N + 1) spectrum values
+
+
+// e / rows for e < 2^20, rows <= 2^12: (e + 1/2) / rows is at least 1/(2 rows)
+// away from an integer, far above the float rounding of the product
+__device__ __forceinline__ int div_rows(int e, float inv_rows) { return (int)(((float)e + 0.5f) * inv_rows); }
+
+// spectrum index of prefetch slot c of tile t (slots past the tile repeat
+// its last value, so every lane loads every slot unconditionally)
+__device__ __forceinline__ long long inv_slot(const StepBuffers& b, const int4& tt, int c) {
+  const ChanDesc* __restrict__ cd = b.chans + tt.x;
+  const int N = cd->nx >> 1, rows = tt.z;
+  const int e = min((int)threadIdx.x + c * FFT_WG, rows * (N + 1) - 1);
+  const int k = div_rows(e, 1.0f / (float)rows), r = e - k * rows;
+  return cd->spec_off + spec_idx(cd->ny, tt.y + r, k);
+}
+
+__global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, int n_tiles) {
+  extern __shared__ double2 lds[];
+  const int max_elems = FFT_PF * FFT_WG;
+  double2* twl = lds + max_elems;            // quadrant twiddle table (N/2 <= max_elems/2 values)
+  double2* side = twl + (max_elems >> 1);    // X[N] of each row of the tile
+  int tw_chan = -1;
+  double2 p0, p1, p2, p3, p4, p5, p6, p7, p8;
+#define EKPNP_INV_PREFETCH(T)                                  \
+  {                                                            \
+    const int4 tn = b.fft_tiles[T];                            \
+    p0 = b.spec[inv_slot(b, tn, 0)];                           \
+    p1 = b.spec[inv_slot(b, tn, 1)];                           \
+    p2 = b.spec[inv_slot(b, tn, 2)];                           \
+    p3 = b.spec[inv_slot(b, tn, 3)];                           \
+    p4 = b.spec[inv_slot(b, tn, 4)];                           \
+    p5 = b.spec[inv_slot(b, tn, 5)];                           \
+    p6 = b.spec[inv_slot(b, tn, 6)];                           \
+    p7 = b.spec[inv_slot(b, tn, 7)];                           \
+    p8 = b.spec[inv_slot(b, tn, 8)];                           \
+  }
+  if ((int)blockIdx.x < n_tiles) EKPNP_INV_PREFETCH(blockIdx.x)
+  for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int4 tt = b.fft_tiles[t];
+    const ChanDesc* __restrict__ cd = b.chans + tt.x;
+    const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
+    const int y0 = tt.y, rows = tt.z;
+    const bool active = cd->active != 0;
+    double* __restrict__ ps = b.psi + cd->fld_off;
+    if (active) {
+      if (tt.x != tw_chan) {
+        const double2* __restrict__ tw = b.tw + cd->tw_off;
+        for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
+        tw_chan = tt.x;
+      }
+      // Dirichlet rows psi(0), psi(ny-1) (odd_extract, poisson.cu:297-303),
+      // written by the tiles holding rows 1 and ny-2
+      const double* __restrict__ Wl = b.walls + cd->wall_off;
+      const long long ws = b.walls_stride;
+      if (y0 == 1)
+        for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[x] = Wl[W_PSI_LO * ws + x];
+      if (y0 + rows - 1 == cd->ny - 2)
+        for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[(long long)(cd->ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
+      // raw spectrum into the rows (X[N] of each row aside)
+      const float inv_rows = 1.0f / (float)rows;
+      const double2 pv[FFT_PF_INV] = {p0, p1, p2, p3, p4, p5, p6, p7, p8};
+#pragma unroll
+      for (int c = 0; c < FFT_PF_INV; ++c) {
+        const int e = threadIdx.x + c * FFT_WG;
+        if (e < rows * (N + 1)) {
+          const int k = div_rows(e, inv_rows), r = e - k * rows;
+          if (k < N) lds[(r << lgN) + k] = pv[c];
+          else side[r] = pv[c];
+        }
+      }
+    }
+    if (t + (int)gridDim.x < n_tiles) EKPNP_INV_PREFETCH(t + gridDim.x)
+    if (active) {
+      __syncthreads();
+      // merge pairs (k, N-k) in place: z[k] = E + i O,
+      // E = (X[k] + conj X[N-k]) / 2, O = (X[k] - conj X[N-k]) / 2 * conj W^k
+      for (int e = threadIdx.x; e < rows * (N / 2 + 1); e += FFT_WG) {
+        const int r = e / (N / 2 + 1), k = e - r * (N / 2 + 1);
+        double2* row = lds + (r << lgN);
+        const double2 a = row[k];
+        const double2 bb = (k == 0) ? side[r] : row[N - k];
+        const double2 xa = cconj(bb);
+        const double2 E = make_double2(0.5 * (a.x + xa.x), 0.5 * (a.y + xa.y));
+        const double2 O = cmul(make_double2(0.5 * (a.x - xa.x), 0.5 * (a.y - xa.y)), cconj(tw_half(twl, k, quarter)));
+        const double2 zk = make_double2(E.x - O.y, E.y + O.x);
+        if (k > 0 && k < N / 2) {
+          // partner N - k from (X[N-k], conj X[k])
+          const double2 xb = cconj(a);
+          const double2 E2 = make_double2(0.5 * (bb.x + xb.x), 0.5 * (bb.y + xb.y));
+          const double2 O2 = cmul(make_double2(0.5 * (bb.x - xb.x), 0.5 * (bb.y - xb.y)),
+                                  cconj(tw_half(twl, N - k, quarter)));
+          row[N - k] = make_double2(E2.x - O2.y, E2.y + O2.x);
+        }
+        row[k] = zk;
+      }
+      __syncthreads();
+      stockham<+1>(lds, rows, N, lgN, twl, lgnx);
+      const double invN = 1.0 / (double)N;
+      for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
+        const int r = e >> lgN, n = e & (N - 1);
+        const double2 z = lds[e];
+        double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
+        *out = make_double2(z.x * invN, z.y * invN);
+      }
+      __syncthreads();
+    }
+  }
+#undef EKPNP_INV_PREFETCH
+}
+
+// ---------------------------------------------------------------------------
 // k_fft_inv: spectrum rows -> psi rows (normalised by 1/nx)
 // ---------------------------------------------------------------------------
 template <bool POW2, bool TWL>
@@ -1074,27 +1331,36 @@ __global__ __launch_bounds__(256) void k_reduce_final(StepBuffers b) {
 // (delta += w_i (ref_old - ref_new), in FP64, rounded once).
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_row_means(StepBuffers b, double* __restrict__ out) {
-  __shared__ double red[4][3];
+  __shared__ double red[4][REF_ROWS];
   const int2 t = b.row_tiles[blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
-  const int nx = cd->nx, ny = cd->ny, y = t.y;
+  const int nx = cd->nx, y = t.y;
   const long long o = cd->fld_off + (long long)y * nx, st = b.macro_stride;
-  double v[3] = {0.0, 0.0, 0.0};
+  double v[REF_ROWS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int x = threadIdx.x; x < nx; x += 256) {
-    v[0] += b.macro[M_RHO * st + o + x];
-    v[1] += b.macro[M_CP * st + o + x];
-    v[2] += b.macro[M_CN * st + o + x];
+    const double r = b.macro[M_RHO * st + o + x], u = b.macro[M_UX * st + o + x];
+    const double cp = b.macro[M_CP * st + o + x], cn = b.macro[M_CN * st + o + x];
+    v[0] += r;
+    v[1] += cp;
+    v[2] += cn;
+    v[3] += r * u;
+    v[4] += cp * u;
+    v[5] += cn * u;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < REF_ROWS; ++k) {
     const double s = wave_sum(v[k]);
     if (lane == 0) red[wv][k] = s;
   }
   __syncthreads();
-  if (threadIdx.x < 3) {
+  if (threadIdx.x < REF_ROWS) {
     const double s = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
-    out[cd->ref_off + (long long)threadIdx.x * ny + y] = s / (double)nx;
+    // wall rows store reversed (fluid) or wall-rule (ions) populations whose
+    // x-flux is not the row's macroscopic one: no flux reference there
+    const bool wall = (y == 0 || y == cd->ny - 1);
+    out[cd->ref_off + (long long)REF_STRIDE * y + threadIdx.x] =
+        (wall && threadIdx.x >= 3) ? 0.0 : s / (double)nx;
   }
 }
 
@@ -1110,14 +1376,16 @@ __global__ __launch_bounds__(LBM_WG) void k_rebase(StepBuffers b, S* __restrict_
   S* __restrict__ P = (cd->swap ? other : cur) + cd->pop_off;
   const double* __restrict__ R0 = b.refs + cd->ref_off;
   const double* __restrict__ R1 = new_refs + cd->ref_off;
+  const double tc = cd->inv_cs2 * cd->inv_cfl;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const double d = R0[s * ny + y] - R1[s * ny + y];
+    const double d = R0[REF_STRIDE * y + s] - R1[REF_STRIDE * y + s];
+    const double dj = (R0[REF_STRIDE * y + 3 + s] - R1[REF_STRIDE * y + 3 + s]) * tc;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const double w = (k == 0) ? W0 : (k < 5 ? WS : WD);
       S* p = P + (long long)(9 * s + k) * cd->plane + node;
-      *p = (S)((double)*p + w * d);
+      *p = (S)((double)*p + w * (d + CX[k] * dj));
     }
   }
 }
@@ -1182,7 +1450,9 @@ hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, co
 // The FFT tile table holds the power-of-two tiles first (n_fft_pow2 of them),
 // then the direct-DFT tiles.
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0) {
+  if (b.n_fft_pow2 > 0 && b.fft_persist) {
+    hipLaunchKernelGGL(k_fft_fwd_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.n_fft_pow2);
+  } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
       hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
     else
@@ -1195,7 +1465,9 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
 }
 
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0) {
+  if (b.n_fft_pow2 > 0 && b.fft_persist) {
+    hipLaunchKernelGGL(k_fft_inv_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.n_fft_pow2);
+  } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
       hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
     else

@@ -1,10 +1,19 @@
 // ekpnp_main.cpp -- the reference's main.cu time loop (main.cu:28-307) on the
-// C ABI: create -> initialization -> init_equilibrium -> N x (stream_collide_save,
-// fast_Poisson) with periodic diagnostics -> performance print -> data_end.dat.
+// C ABI: create -> initialization (or read_data restart) -> init_equilibrium
+// -> N x (stream_collide_save, fast_Poisson) with the reference's periodic
+// outputs -> performance print -> data_end.dat.
 //
 // usage: ekpnp_run [--config ref|c0|c1] [--steps N] [--fp32|--fp64]
-//                  [--print-every K] [--out data_end.dat]
+//                  [--nsave K] [--ndmd K] [--print-current K]
+//                  [--restart data_end.dat] [--checkpoint FILE] [--resume FILE]
+//                  [--dir OUTDIR]
+// Outputs (in OUTDIR, default "."): data.dat (Tecplot, every nsave steps),
+// charge_data (every ndmd), umax.dat and the printed wall current (every
+// print-current), data_end.dat at the end (LBM.h:115-119 intervals by
+// default).
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -25,45 +34,38 @@ namespace {
     }                                                                                \
   } while (0)
 
-// wall current I = K dy sum_x c(x, NY-1) Ey(x, NY-1), with c at the two wall
-// rows linearly extrapolated from the interior first (LBM.cu:1394-1417)
-double wall_current(const ekpnp_channel_params& p, std::vector<double> c, const std::vector<double>& ey) {
-  const int nx = p.nx, ny = p.ny;
-  for (int x = 0; x < nx; ++x) {
-    c[x] = 2.0 * c[(size_t)nx + x] - c[2 * (size_t)nx + x];
-    c[(size_t)(ny - 1) * nx + x] = 2.0 * c[(size_t)(ny - 2) * nx + x] - c[(size_t)(ny - 3) * nx + x];
-  }
-  double I = 0.0;
-  for (int x = 0; x < nx; ++x) I += c[(size_t)(ny - 1) * nx + x] * ey[(size_t)(ny - 1) * nx + x];
-  return I * p.mob_pos * p.dy;
-}
-
-// max u_y after extrapolating the top row (LBM.cu:1419-1451)
-double max_uy(const ekpnp_channel_params& p, std::vector<double> uy) {
-  const int nx = p.nx, ny = p.ny;
-  for (int x = 0; x < nx; ++x)
-    uy[(size_t)(ny - 1) * nx + x] = 2.0 * uy[(size_t)(ny - 2) * nx + x] - uy[(size_t)(ny - 3) * nx + x];
-  double m = 0.0;
-  for (double v : uy) m = v > m ? v : m;
-  return m;
+// first iteration index j >= i with j % k == 1 (the reference's output test)
+long next_event(long i, long k) {
+  if (k < 2) return LONG_MAX;  // never fires
+  long r = i % k;
+  return r <= 1 ? i + (1 - r) : i + (k - r) + 1;
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-  std::string config = "ref", out = "data_end.dat";
-  long nsteps = -1, print_every = 2000;
+  std::string config = "ref", dir = ".", restart, ckpt_out, resume;
+  long nsteps = -1, nsave = -1, ndmd = -1, print_current = -1;
   int precision = -1;
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
-    if (a == "--config" && i + 1 < argc) config = argv[++i];
-    else if (a == "--steps" && i + 1 < argc) nsteps = atol(argv[++i]);
+    auto next = [&]() { return std::string(i + 1 < argc ? argv[++i] : ""); };
+    if (a == "--config") config = next();
+    else if (a == "--steps") nsteps = atol(next().c_str());
     else if (a == "--fp32") precision = EKPNP_POP_FP32;
     else if (a == "--fp64") precision = EKPNP_POP_FP64;
-    else if (a == "--print-every" && i + 1 < argc) print_every = atol(argv[++i]);
-    else if (a == "--out" && i + 1 < argc) out = argv[++i];
+    else if (a == "--nsave") nsave = atol(next().c_str());
+    else if (a == "--ndmd") ndmd = atol(next().c_str());
+    else if (a == "--print-current") print_current = atol(next().c_str());
+    else if (a == "--restart") restart = next();
+    else if (a == "--checkpoint") ckpt_out = next();
+    else if (a == "--resume") resume = next();
+    else if (a == "--dir") dir = next();
     else {
-      fprintf(stderr, "usage: %s [--config ref|c0|c1] [--steps N] [--fp32|--fp64] [--print-every K] [--out FILE]\n", argv[0]);
+      fprintf(stderr,
+              "usage: %s [--config ref|c0|c1] [--steps N] [--fp32|--fp64] [--nsave K] [--ndmd K] "
+              "[--print-current K] [--restart data_end.dat] [--checkpoint FILE] [--resume FILE] [--dir D]\n",
+              argv[0]);
       return 2;
     }
   }
@@ -72,7 +74,7 @@ int main(int argc, char** argv) {
   uint64_t seed = 0;
   if (config == "ref") {
     ekpnp_reference_params(&p);
-    mode = EKPNP_INIT_REFERENCE;
+    mode = EKPNP_INIT_PB_PICARD;  // initialization(), LBM.cu:95-145
     if (nsteps < 0) nsteps = 100000;  // LBM.h:115
     if (precision < 0) precision = EKPNP_POP_FP64;
   } else if (config == "c0") {
@@ -88,6 +90,13 @@ int main(int argc, char** argv) {
     fprintf(stderr, "unknown config %s\n", config.c_str());
     return 2;
   }
+  // LBM.h:116-119; NDMD's literal 5000000000000 wraps to 658067456 in the
+  // reference's unsigned int, so charge_data is written once, after step 1
+  if (nsave < 0) nsave = nsteps / 5;
+  if (ndmd < 0) ndmd = 658067456L;
+  if (print_current < 0) print_current = 2000;
+  const std::string f_tec = dir + "/data.dat", f_umax = dir + "/umax.dat", f_dmd = dir + "/charge_data",
+                    f_end = dir + "/data_end.dat";
   printf("Simulating EK-PNP in 2D\n      domain size: %dx%d\n        timesteps: %ld\n\n", p.nx, p.ny, nsteps);
 
   ekpnp_options o;
@@ -99,25 +108,42 @@ int main(int argc, char** argv) {
     fprintf(stderr, "ekpnp_create failed (%d): %s\n", rc, ekpnp_create_error());
     return 1;
   }
-  CHECK(ekpnp_initialization(ctx, mode, seed));
-  CHECK(ekpnp_init_equilibrium(ctx));
+  double t = 0.0;
+  if (!resume.empty()) {
+    CHECK(ekpnp_load_checkpoint(ctx, resume.c_str()));
+    t = p.dt * (double)ekpnp_steps_done(ctx);
+  } else {
+    CHECK(ekpnp_initialization(ctx, mode, seed));
+    if (!restart.empty()) CHECK(ekpnp_read_data(ctx, 0, restart.c_str(), &t));  // main.cu:165-167
+    CHECK(ekpnp_init_equilibrium(ctx));                                          // main.cu:174
+  }
+  CHECK(ekpnp_save_tecplot(ctx, 0, t, f_tec.c_str(), 1));  // main.cu:178
+  std::remove(f_umax.c_str());
 
   const size_t nn = (size_t)p.nx * p.ny;
-  std::vector<double> c(nn), ey(nn), uy(nn);
   auto t0 = std::chrono::steady_clock::now();
-  // the reference runs NSTEPS+1 iterations and counts NSTEPS (main.cu:190, 248)
-  long done = 0;
-  while (done <= nsteps) {
-    long chunk = print_every - (done % print_every);
-    if (done + chunk > nsteps + 1) chunk = nsteps + 1 - done;
-    CHECK(ekpnp_step(ctx, (int)chunk));
-    done += chunk;
-    if (done % print_every == 1 || done == nsteps + 1) {
-      CHECK(ekpnp_get_field(ctx, 0, EKPNP_FIELD_CPOS, c.data()));
-      CHECK(ekpnp_get_field(ctx, 0, EKPNP_FIELD_EY, ey.data()));
-      CHECK(ekpnp_get_field(ctx, 0, EKPNP_FIELD_UY, uy.data()));
-      printf("Iteration: %ld, Current = %g, max uy = %g\n", done - 1, wall_current(p, c, ey), max_uy(p, uy));
+  // iterations i = 0..NSTEPS (main.cu:190); after iteration i the reference
+  // writes when i % K == 1, then advances t
+  long i = 0;
+  while (i <= nsteps) {
+    long stop = nsteps;
+    for (long k : {nsave, ndmd, print_current}) stop = std::min(stop, next_event(i, k));
+    CHECK(ekpnp_step(ctx, (int)(stop - i + 1)));
+    t += p.dt * (double)(stop - i);  // t of iteration `stop`
+    i = stop;
+    if (nsave > 0 && i % nsave == 1) CHECK(ekpnp_save_tecplot(ctx, 0, t, f_tec.c_str(), 0));
+    if (ndmd > 0 && i % ndmd == 1) {
+      printf("Iteration: %ld, physical time: %g.\n", i, t);
+      CHECK(ekpnp_save_charge_data(ctx, 0, f_dmd.c_str()));
     }
+    if (print_current > 0 && i % print_current == 1) {
+      double I = 0.0, umax = 0.0;
+      CHECK(ekpnp_current(ctx, 0, &I));
+      printf("Iteration: %ld, physical time: %g, Current = %g\n", i, t, I);
+      CHECK(ekpnp_record_umax(ctx, 0, t, f_umax.c_str(), &umax));
+    }
+    t += p.dt;
+    ++i;
   }
   CHECK(ekpnp_synchronize(ctx));
   auto t1 = std::chrono::steady_clock::now();
@@ -130,26 +156,8 @@ int main(int argc, char** argv) {
   printf("           clock runtime: %.3f (s)\n", runtime);
   printf("                   speed: %.2f (Mlups)\n", mlups);
   printf("  algorithmic bandwidth: %.1f (GB/s)\n", bytes / runtime / 1e9);
-
-  // data_end.dat: t ux uy rho c phi Ex Ey T per node, x fastest (LBM.cu:1294-1302),
-  // written with 17 significant digits so that it restarts exactly
-  std::vector<std::vector<double>> f(8, std::vector<double>(nn));
-  const int order[8] = {EKPNP_FIELD_UX, EKPNP_FIELD_UY, EKPNP_FIELD_RHO, EKPNP_FIELD_CPOS,
-                        EKPNP_FIELD_PSI, EKPNP_FIELD_EX, EKPNP_FIELD_EY, EKPNP_FIELD_CNEG};
-  for (int k = 0; k < 8; ++k) CHECK(ekpnp_get_field(ctx, 0, order[k], f[k].data()));
-  FILE* fend = fopen(out.c_str(), "w");
-  if (!fend) {
-    fprintf(stderr, "cannot open %s\n", out.c_str());
-    ekpnp_destroy(ctx);
-    return 1;
-  }
-  const double t = p.dt * (double)(nsteps + 1);
-  for (size_t i = 0; i < nn; ++i) {
-    fprintf(fend, "%.17g", t);
-    for (int k = 0; k < 8; ++k) fprintf(fend, " %.17g", f[k][i]);
-    fputc('\n', fend);
-  }
-  fclose(fend);
+  CHECK(ekpnp_save_data_end(ctx, 0, t, f_end.c_str()));
+  if (!ckpt_out.empty()) CHECK(ekpnp_save_checkpoint(ctx, ckpt_out.c_str()));
   ekpnp_destroy(ctx);
   return 0;
 }

@@ -112,7 +112,7 @@ SIGNATURES = [
     ("ekpnp_save_checkpoint", ctypes.c_int, [_VP, ctypes.c_char_p]),
     ("ekpnp_load_checkpoint", ctypes.c_int, [_VP, ctypes.c_char_p]),
     ("ekpnp_get_params", ctypes.c_int, [_VP, ctypes.c_int, _P(ChannelParams)]),
-    ("ekpnp_set_poisson_plan", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    ("ekpnp_set_poisson_plan", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("ekpnp_autotune", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     ("ekpnp_num_channels", ctypes.c_int, [_VP]),
     ("ekpnp_steps_done", ctypes.c_longlong, [_VP]),
@@ -228,7 +228,7 @@ class Solver:
     """One EK-PNP context: every channel of `channels` advances in lock-step."""
 
     def __init__(self, channels: Sequence[ChannelParams], precision: str = "fp32", device: int = 0,
-                 use_graph: bool = True, compute_fp64: bool = False):
+                 use_graph: bool = True, compute_fp64: bool = True):
         L = lib()
         if isinstance(channels, ChannelParams):
             channels = [channels]
@@ -368,13 +368,14 @@ class Solver:
     def load_checkpoint(self, path: str) -> None:
         self._check(lib().ekpnp_load_checkpoint(self._ctx, os.fsencode(path)))
 
-    def set_poisson_plan(self, fft_elems: int = 2048, tri_rows: int = 16) -> None:
-        self._check(lib().ekpnp_set_poisson_plan(self._ctx, int(fft_elems), int(tri_rows)))
+    def set_poisson_plan(self, fft_elems: int = 2048, tri_rows: int = 16, persistent: bool = True) -> None:
+        self._check(lib().ekpnp_set_poisson_plan(self._ctx, int(fft_elems), int(tri_rows), int(bool(persistent))))
 
     def autotune(self, reps: int = 20) -> dict:
-        out = np.zeros(3, dtype=np.float64)
+        out = np.zeros(4, dtype=np.float64)
         self._check(lib().ekpnp_autotune(self._ctx, int(reps), _dptr(out)))
-        return {"fft_elems": int(out[0]), "tri_rows": int(out[1]), "poisson_ms": float(out[2])}
+        return {"fft_elems": int(out[0]), "tri_rows": int(out[1]), "poisson_ms": float(out[2]),
+                "persistent": bool(out[3])}
 
     def get_params(self, ch: int) -> ChannelParams:
         out = ChannelParams()

@@ -171,21 +171,17 @@ def hs_scale(p: ChannelParams) -> float:
     return p.e_applied * (-p.psi_lo) * (p.force_coef / p.poisson_coef) / (p.rho0 * p.nu)
 
 
-def targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:
-    """Normalised 1024-vector from x-averaged wall-normal profiles (length ny)."""
+def _normalized(p: ChannelParams, psi, cpos, cneg, ux):
+    """The four profiles on the channel's own rows, each O(1)."""
     z = p.psi_lo
-    out = np.concatenate([resample(psi / z, N_PTS),
-                          resample(np.log(np.maximum(cpos, 1e-300)) / (-z), N_PTS),
-                          resample(np.log(np.maximum(cneg, 1e-300)) / z, N_PTS),
-                          resample(ux / hs_scale(p), N_PTS)])
-    return out.astype(np.float32)
+    return [np.asarray(psi, dtype=np.float64) / z,
+            np.log(np.maximum(cpos, 1e-300)) / (-z),
+            np.log(np.maximum(cneg, 1e-300)) / z,
+            np.asarray(ux, dtype=np.float64) / hs_scale(p)]
 
 
-def profiles(p: ChannelParams, y_norm: np.ndarray):
-    """Inverse of targets(): (psi, c+, c-, u_x) on the channel's ny rows."""
+def _denormalized(p: ChannelParams, parts):
     z = p.psi_lo
-    ny = p.ny
-    parts = [resample(y_norm[i * N_PTS:(i + 1) * N_PTS].astype(np.float64), ny) for i in range(N_FIELDS)]
     psi = parts[0] * z
     cpos = np.exp(parts[1] * (-z))
     cneg = np.exp(parts[2] * z)
@@ -195,6 +191,20 @@ def profiles(p: ChannelParams, y_norm: np.ndarray):
     cpos[0], cpos[-1] = p.cpos_lo, p.cpos_hi
     cneg[0], cneg[-1] = p.cneg_lo, p.cneg_hi
     return psi, cpos, cneg, ux
+
+
+def _split(y_norm):
+    return [np.asarray(y_norm[i * N_PTS:(i + 1) * N_PTS], dtype=np.float64) for i in range(N_FIELDS)]
+
+
+def targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:
+    """Normalised 1024-vector from x-averaged wall-normal profiles (length ny)."""
+    return np.concatenate([resample(a, N_PTS) for a in _normalized(p, psi, cpos, cneg, ux)]).astype(np.float32)
+
+
+def profiles(p: ChannelParams, y_norm: np.ndarray):
+    """Inverse of targets(): (psi, c+, c-, u_x) on the channel's ny rows."""
+    return _denormalized(p, [resample(a, p.ny) for a in _split(y_norm)])
 
 
 # ---------------------------------------------------------------------------
@@ -259,13 +269,19 @@ def baseline(p: ChannelParams):
 
 
 def residual_targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:
-    """targets() of a steady state minus targets() of the physics baseline."""
-    return targets(p, psi, cpos, cneg, ux) - targets(p, *baseline(p))
+    """Normalised profiles of a steady state minus those of the physics
+    baseline, subtracted on the channel's own rows and then resampled (only
+    the smooth difference is interpolated)."""
+    have = _normalized(p, psi, cpos, cneg, ux)
+    base = _normalized(p, *baseline(p))
+    return np.concatenate([resample(a - b, N_PTS) for a, b in zip(have, base)]).astype(np.float32)
 
 
 def residual_profiles(p: ChannelParams, y_norm: np.ndarray):
-    """Inverse of residual_targets()."""
-    return profiles(p, y_norm + targets(p, *baseline(p)))
+    """Inverse of residual_targets(): baseline on the rows plus the resampled
+    correction."""
+    base = _normalized(p, *baseline(p))
+    return _denormalized(p, [b + resample(a, p.ny) for a, b in zip(_split(y_norm), base)])
 
 
 def channel_of(zeta, kappa_h, e_x, nx, ny, tau=0.8, diff=0.05) -> ChannelParams:

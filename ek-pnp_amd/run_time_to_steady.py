@@ -44,19 +44,36 @@ def xavg(s, ch, name):
     return s.get_field(ch, name).mean(axis=1)
 
 
-def run_cold(channels, precision, seed, tol, every, max_steps):
+def run_to_residual(s, tol, every, max_steps, tag, chunk=100_000):
+    """run_to_residual in slices of `chunk` steps with a progress line per
+    slice; a channel's count is the step at which it first met tol (channels
+    that met it are stepped again briefly in later slices, until the first
+    check re-freezes them)."""
+    n = len(s.channels)
+    done = np.full(n, -1, dtype=np.int64)
+    start = s.steps_done
+    while (done < 0).any() and s.steps_done - start < max_steps:
+        off = s.steps_done - start
+        res = s.run_to_residual(tol, every, min(chunk, max_steps - off))
+        new = (done < 0) & (res >= 0)
+        done[new] = off + res[new]
+        log(f"{tag}: {int((done >= 0).sum())}/{n} at residual {tol:g} after {s.steps_done - start} steps")
+    return done
+
+
+def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold"):
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.synchronize()
     t0 = time.perf_counter()
-    steps = s.run_to_residual(tol, every, max_steps)
+    steps = run_to_residual(s, tol, every, max_steps, tag)
     s.synchronize()
     return s, steps, time.perf_counter() - t0
 
 
 def generate(channels, precision, seed, tol, every, max_steps, residual):
-    s, steps, wall = run_cold(channels, precision, seed, tol, every, max_steps)
+    s, steps, wall = run_cold(channels, precision, seed, tol, every, max_steps, tag="training data")
     X = S.channel_features(channels)
     tf = S.residual_targets if residual else S.targets
     T = np.stack([tf(p, xavg(s, c, "psi"), xavg(s, c, "cpos"), xavg(s, c, "cneg"), xavg(s, c, "ux"))
@@ -130,7 +147,7 @@ def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, a
         s.set_fields(c, f["rho"], f["ux"], f["uy"], f["cpos"], f["cneg"])
     if rejected:
         s.init_equilibrium()
-    steps = s.run_to_residual(tol, every, max_steps)
+    steps = run_to_residual(s, tol, every, max_steps, "warm")
     s.synchronize()
     wall = time.perf_counter() - t0
     total = np.where(steps >= 0, steps + probe, -1)

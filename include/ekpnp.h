@@ -261,14 +261,18 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
 /* ---- work decomposition ---------------------------------------------------- */
 /* Poisson plan: x-FFT work-groups hold up to fft_elems complex values (whole
  * rows; default 2048), tridiagonal columns are cut into chunks of about
- * tri_rows rows (default 16).  Results do not depend on fft_elems; tri_rows
- * changes the FP64 rounding of psi only (chunk boundaries). */
-int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows);
+ * tri_rows rows (default 16); fft_persistent != 0 selects the persistent,
+ * software-pipelined FFT kernels where the tiles allow them (fft_elems <=
+ * 2048, nx <= 4096).  Results do not depend on fft_elems or the kernel
+ * choice; tri_rows changes the FP64 rounding of psi only (chunk
+ * boundaries). */
+int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent);
 /* Times the Poisson stages over these channels for fft_elems in {1024,
- * 2048, 4096} and tri_rows in {8, 16} (reps solves each, HIP events on the
- * context's stream), keeps the fastest, and restores psi.  out (may be NULL)
- * receives {fft_elems, tri_rows, ms per solve}.  No counterpart in the
- * reference (cuFFT plans are fixed by main.cu:113). */
+ * 2048, 4096}, both FFT kernel families and tri_rows in {8, 16} (reps solves
+ * each, HIP events on the context's stream), keeps the fastest, and restores
+ * psi.  out (may be NULL, else 4 doubles) receives {fft_elems, tri_rows, ms
+ * per solve, persistent}.  No counterpart in the reference (cuFFT plans are
+ * fixed by main.cu:113). */
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
 
 /* ---- introspection / timing -------------------------------------------------- */

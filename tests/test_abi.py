@@ -94,3 +94,14 @@ def test_create_rejects_bad_parameters_before_touching_a_device():
     arr = (ekpnp.ChannelParams * 1)(big)
     assert lib.ekpnp_create(arr, 1, None, ctypes.byref(ctx)) == ekpnp.ERR_UNSUPPORTED
     assert lib.ekpnp_create(arr, 0, None, ctypes.byref(ctx)) == ekpnp.ERR_INVALID
+
+
+def test_python_defaults_match_the_c_defaults():
+    # the binding's Solver must not silently pick other options than
+    # ekpnp_default_options (FP64 collision arithmetic, graphs on)
+    import inspect
+    o = ekpnp.Options()
+    ekpnp.lib().ekpnp_default_options(ctypes.byref(o))
+    sig = inspect.signature(ekpnp.Solver.__init__).parameters
+    assert o.compute_fp64 == 1 and sig["compute_fp64"].default is True
+    assert o.use_graph == 1 and sig["use_graph"].default is True
