@@ -262,8 +262,10 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
       if (d.pow2) {
         const int N = d.nx / 2;
         rows = std::max(1, std::min(M, fft_elems / N));
-        lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(rows * N * sizeof(double2)));
-        lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((rows * N + d.nx / 4) * sizeof(double2)));
+        // rows are padded to whole 16-element groups (LDS swizzle)
+        const int padded = (rows * N + 15) & ~15;
+        lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(padded * sizeof(double2)));
+        lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((padded + d.nx / 4) * sizeof(double2)));
       } else {
         lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
       }
@@ -275,7 +277,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
     }
     if (pass == 0) n_fft_pow2 = (int)c->fft_tiles.size();
   }
-  int tri_lmax = 16;
+  int tri_lmax = 1;
   for (int i = 0; i < c->nchan; ++i) {
     ChanDesc& d = c->desc[i];
     const int M = d.ny - 2;
@@ -287,8 +289,9 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
     c->ctiles[i].tri_first = (int)c->tri_tiles.size();
     for (int k0 = 0; k0 < d.nk; k0 += d.tri_G) c->tri_tiles.push_back(make_int2(i, k0));
     c->ctiles[i].tri_count = (int)c->tri_tiles.size() - c->ctiles[i].tri_first;
-    if (d.tri_L > 16) tri_lmax = 32;
+    tri_lmax = std::max(tri_lmax, d.tri_L);
   }
+  if (tri_lmax > TRI_LMAX) return hipErrorInvalidValue;  // ny is validated against MAX_INTERIOR_ROWS
   if (c->fft_tiles.size() > c->fft_tiles_cap || c->tri_tiles.size() > c->tri_tiles_cap) return hipErrorInvalidValue;
   hipError_t e = hipMemcpy(c->d_fft_tiles, c->fft_tiles.data(), c->fft_tiles.size() * sizeof(int4),
                            hipMemcpyHostToDevice);

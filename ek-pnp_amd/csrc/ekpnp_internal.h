@@ -26,6 +26,7 @@ constexpr unsigned FFT_LDS_PERSIST = (FFT_PERSIST_ELEMS + FFT_PERSIST_ELEMS / 2 
 constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
+constexpr int TRI_LMAX = 16;              // rows per tridiagonal chunk (one lane)
 constexpr int MAX_INTERIOR_ROWS = 2048;  // 128 chunks x 16 rows per tridiagonal column
 
 struct ChanDesc {
@@ -75,7 +76,7 @@ struct StepBuffers {
   int fft_tw_lds;               // the power-of-two kernels stage twiddles in LDS
   const int2* tri_tiles;        // (channel, first mode) per tridiagonal work-group
   int n_tri_tiles;
-  int tri_lmax;                 // 16 or 32
+  int tri_lmax;                 // longest tridiagonal chunk (<= TRI_LMAX)
   double* rho_e;                // c+ - c- per node (FP64)
   double* psi;                  // potential per node (FP64)
   double* macro;                // M_COUNT planes

@@ -634,6 +634,14 @@ __device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t,
   return SIGN > 0 ? cconj(r) : r;
 }
 
+// LDS position of row element i: bits 0-3 XOR bits 4-7.  The Stockham
+// passes write with strides 8 (first pass) and 8 x 16 (second) elements,
+// which without the swizzle put 8 (resp. 2) of every 16 lanes of a
+// ds_write_b128 on the same banks; with it every 16 consecutive lanes of
+// every read and write pattern here cover 16 distinct 16-byte bank groups.
+// A bijection within each 256-element block; every row access uses it.
+__device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
+
 // ---------------------------------------------------------------------------
 // Stockham autosort FFT of `rows` rows of length N = 2^lgN (4 <= N, and
 // rows * N <= 16 * FFT_WG) held in LDS, in place.  Stage (n, s) with radix R,
@@ -660,7 +668,7 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
       const int p = jj >> lgs, q = jj & (s - 1);
       const int base = (r << lgN) + q + s * p;
 #pragma unroll
-      for (int j = 0; j < R; ++j) v[c][j] = x[base + ((s * j) << lgm)];
+      for (int j = 0; j < R; ++j) v[c][j] = x[sw(base + ((s * j) << lgm))];
     }
   }
   __syncthreads();
@@ -672,36 +680,36 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
       const int p = jj >> lgs, q = jj & (s - 1);
       dft_reg<SIGN, R>(v[c]);
       const int ob = (r << lgN) + q + s * R * p;
-      x[ob] = v[c][0];
+      x[sw(ob)] = v[c][0];
       // W^(p u): table values for u = 1, 2, 4, 8, products (at most three
       // roundings deep) for the rest
       const int t1 = p << (lgnx - lgn);
       const double2 w1 = tw_full<SIGN>(twq, t1, lgnx);
       if (R == 2) {
-        x[ob + s] = cmul(v[c][1], w1);
+        x[sw(ob + s)] = cmul(v[c][1], w1);
       } else {
         const double2 w2 = tw_full<SIGN>(twq, 2 * t1, lgnx);
-        x[ob + s] = cmul(v[c][1], w1);
-        x[ob + 2 * s] = cmul(v[c][2], w2);
+        x[sw(ob + s)] = cmul(v[c][1], w1);
+        x[sw(ob + 2 * s)] = cmul(v[c][2], w2);
         const double2 w3 = cmul(w2, w1);
-        x[ob + 3 * s] = cmul(v[c][3], w3);
+        x[sw(ob + 3 * s)] = cmul(v[c][3], w3);
         if (R >= 8) {
           const double2 w4 = tw_full<SIGN>(twq, 4 * t1, lgnx);
-          x[ob + 4 * s] = cmul(v[c][4], w4);
-          x[ob + 5 * s] = cmul(v[c][5], cmul(w4, w1));
-          x[ob + 6 * s] = cmul(v[c][6], cmul(w4, w2));
-          x[ob + 7 * s] = cmul(v[c][7], cmul(w4, w3));
+          x[sw(ob + 4 * s)] = cmul(v[c][4], w4);
+          x[sw(ob + 5 * s)] = cmul(v[c][5], cmul(w4, w1));
+          x[sw(ob + 6 * s)] = cmul(v[c][6], cmul(w4, w2));
+          x[sw(ob + 7 * s)] = cmul(v[c][7], cmul(w4, w3));
           if (R == 16) {
             const double2 w8 = tw_full<SIGN>(twq, 8 * t1, lgnx);
-            x[ob + 8 * s] = cmul(v[c][8], w8);
-            x[ob + 9 * s] = cmul(v[c][9], cmul(w8, w1));
-            x[ob + 10 * s] = cmul(v[c][10], cmul(w8, w2));
-            x[ob + 11 * s] = cmul(v[c][11], cmul(w8, w3));
+            x[sw(ob + 8 * s)] = cmul(v[c][8], w8);
+            x[sw(ob + 9 * s)] = cmul(v[c][9], cmul(w8, w1));
+            x[sw(ob + 10 * s)] = cmul(v[c][10], cmul(w8, w2));
+            x[sw(ob + 11 * s)] = cmul(v[c][11], cmul(w8, w3));
             const double2 w12 = cmul(w8, w4);
-            x[ob + 12 * s] = cmul(v[c][12], w12);
-            x[ob + 13 * s] = cmul(v[c][13], cmul(w12, w1));
-            x[ob + 14 * s] = cmul(v[c][14], cmul(w12, w2));
-            x[ob + 15 * s] = cmul(v[c][15], cmul(w12, w3));
+            x[sw(ob + 12 * s)] = cmul(v[c][12], w12);
+            x[sw(ob + 13 * s)] = cmul(v[c][13], cmul(w12, w1));
+            x[sw(ob + 14 * s)] = cmul(v[c][14], cmul(w12, w2));
+            x[sw(ob + 15 * s)] = cmul(v[c][15], cmul(w12, w3));
           }
         }
       }
@@ -772,7 +780,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
   double2* __restrict__ spec = b.spec + cd->spec_off;
   if (POW2) {
     const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    double2* twl = lds + rows * N;  // quadrant twiddle table after the rows
+    double2* twl = lds + ((rows * N + 15) & ~15);  // quadrant twiddles after the (swizzled) rows
     if (TWL)
       for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
     const double2* twq = TWL ? twl : tw;
@@ -790,7 +798,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
         v0 -= Wl[W_PSI_HI * ws + 2 * n];
         v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
       }
-      lds[e] = make_double2(v0, v1);
+      lds[sw(e)] = make_double2(v0, v1);
     }
     __syncthreads();
     stockham<-1>(lds, rows, N, lgN, twq, lgnx);
@@ -798,8 +806,8 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
     // rows vary fastest across lanes: 8 modes x `rows` rows are contiguous
     for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
       const int k = e / rows, r = e - k * rows;
-      const double2 zk = lds[(r << lgN) + (k & (N - 1))];
-      const double2 zc = cconj(lds[(r << lgN) + ((N - k) & (N - 1))]);
+      const double2 zk = lds[sw((r << lgN) + (k & (N - 1)))];
+      const double2 zc = cconj(lds[sw((r << lgN) + ((N - k) & (N - 1)))]);
       const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
       const double2 dd = csub(zk, zc);
       const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
@@ -887,7 +895,7 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, int n_ti
             v0 -= Wl[W_PSI_HI * ws + 2 * n];
             v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
           }
-          lds[e] = make_double2(v0, v1);
+          lds[sw(e)] = make_double2(v0, v1);
         }
       }
     }
@@ -899,8 +907,8 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, int n_ti
       double2* __restrict__ spec = b.spec + cd->spec_off;
       for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
         const int k = e / rows, r = e - k * rows;
-        const double2 zk = lds[(r << lgN) + (k & (N - 1))];
-        const double2 zc = cconj(lds[(r << lgN) + ((N - k) & (N - 1))]);
+        const double2 zk = lds[sw((r << lgN) + (k & (N - 1)))];
+        const double2 zc = cconj(lds[sw((r << lgN) + ((N - k) & (N - 1)))]);
         const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
         const double2 dd = csub(zk, zc);
         const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
@@ -978,7 +986,7 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, int n_ti
         const int e = threadIdx.x + c * FFT_WG;
         if (e < rows * (N + 1)) {
           const int k = div_rows(e, inv_rows), r = e - k * rows;
-          if (k < N) lds[(r << lgN) + k] = pv[c];
+          if (k < N) lds[sw((r << lgN) + k)] = pv[c];
           else side[r] = pv[c];
         }
       }
@@ -990,9 +998,9 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, int n_ti
       // E = (X[k] + conj X[N-k]) / 2, O = (X[k] - conj X[N-k]) / 2 * conj W^k
       for (int e = threadIdx.x; e < rows * (N / 2 + 1); e += FFT_WG) {
         const int r = e / (N / 2 + 1), k = e - r * (N / 2 + 1);
-        double2* row = lds + (r << lgN);
-        const double2 a = row[k];
-        const double2 bb = (k == 0) ? side[r] : row[N - k];
+        const int rb = r << lgN;
+        const double2 a = lds[sw(rb + k)];
+        const double2 bb = (k == 0) ? side[r] : lds[sw(rb + N - k)];
         const double2 xa = cconj(bb);
         const double2 E = make_double2(0.5 * (a.x + xa.x), 0.5 * (a.y + xa.y));
         const double2 O = cmul(make_double2(0.5 * (a.x - xa.x), 0.5 * (a.y - xa.y)), cconj(tw_half(twl, k, quarter)));
@@ -1003,16 +1011,16 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, int n_ti
           const double2 E2 = make_double2(0.5 * (bb.x + xb.x), 0.5 * (bb.y + xb.y));
           const double2 O2 = cmul(make_double2(0.5 * (bb.x - xb.x), 0.5 * (bb.y - xb.y)),
                                   cconj(tw_half(twl, N - k, quarter)));
-          row[N - k] = make_double2(E2.x - O2.y, E2.y + O2.x);
+          lds[sw(rb + N - k)] = make_double2(E2.x - O2.y, E2.y + O2.x);
         }
-        row[k] = zk;
+        lds[sw(rb + k)] = zk;
       }
       __syncthreads();
       stockham<+1>(lds, rows, N, lgN, twl, lgnx);
       const double invN = 1.0 / (double)N;
       for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
         const int r = e >> lgN, n = e & (N - 1);
-        const double2 z = lds[e];
+        const double2 z = lds[sw(e)];
         double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
         *out = make_double2(z.x * invN, z.y * invN);
       }
@@ -1048,7 +1056,7 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
   }
   if (POW2) {
     const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    double2* twl = lds + rows * N;
+    double2* twl = lds + ((rows * N + 15) & ~15);
     if (TWL) {
       for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
       __syncthreads();
@@ -1060,14 +1068,14 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base
       const double2 xc = cconj(spec[spec_idx(cd->ny, y0 + r, N - k)]);
       const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
       const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw_half(twq, k, quarter)));
-      lds[(r << lgN) + k] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
+      lds[sw((r << lgN) + k)] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
     }
     __syncthreads();
     stockham<+1>(lds, rows, N, lgN, twq, lgnx);
     const double invN = 1.0 / (double)N;
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
       const int r = e >> lgN, n = e & (N - 1);
-      const double2 z = lds[e];
+      const double2 z = lds[sw(e)];
       double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
       *out = make_double2(z.x * invN, z.y * invN);
     }
@@ -1480,10 +1488,8 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
 }
 
 hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s) {
-  if (b.tri_lmax <= 16)
-    hipLaunchKernelGGL((k_tridiag<16>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
-  else
-    hipLaunchKernelGGL((k_tridiag<32>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
+  // the plan keeps every chunk at <= TRI_LMAX rows
+  hipLaunchKernelGGL((k_tridiag<TRI_LMAX>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
   return hipGetLastError();
 }
 

@@ -1,10 +1,11 @@
 """Warm-start surrogate: channel parameters -> wall-normal steady profiles.
 
-The network (include/ekpnp_mlp.h) maps four features of a channel
-(zeta, kappa h, E_x, n_y) to psi, c+, c- and u_x at N_PTS points spread
-uniformly over the wall-normal coordinate y in [0, ny-1].  It is trained and
-evaluated by the library's BF16 MFMA kernels; this module only prepares
-features/targets, resamples profiles and drives training.
+The network (include/ekpnp_mlp.h) maps a channel's (zeta, kappa h, E_x,
+n_y), presented as the six derived features of features(), to psi, c+, c-
+and u_x at N_PTS points spread uniformly over the wall-normal coordinate y
+in [0, ny-1].  It is trained and evaluated by the library's BF16 MFMA
+kernels; this module only prepares features/targets, resamples profiles,
+supplies the physics baseline and drives training.
 
 Targets are normalised so that every output is O(1) across the parameter
 range:
@@ -26,7 +27,7 @@ from . import ChannelParams, EkpnpError, lib
 N_PTS = 256
 N_FIELDS = 4          # psi, c+, c-, u_x
 N_OUT = N_PTS * N_FIELDS
-N_IN = 4              # zeta, kappa h, E_x, n_y
+N_IN = 6              # from (zeta, kappa h, E_x, n_y); see features()
 
 
 class MLPConfig(ctypes.Structure):
@@ -76,6 +77,15 @@ def _fp(a):
     return a.ctypes.data_as(_FP)
 
 
+def _batch(a, width: int, what: str) -> np.ndarray:
+    """float32 [B, width] copy of a; raises on any other shape (the C side
+    reads exactly B x width values)."""
+    arr = np.ascontiguousarray(a, dtype=np.float32)
+    if arr.ndim != 2 or arr.shape[1] != width:
+        raise ValueError(f"{what} must have shape [batch, {width}], got {arr.shape}")
+    return arr
+
+
 class MLP:
     """GPU multilayer perceptron (BF16 MFMA, FP32 accumulation, Adam)."""
 
@@ -120,7 +130,7 @@ class MLP:
         self._check(_lib().ekpnp_mlp_set_params(self._h, _fp(arr)))
 
     def forward(self, x) -> np.ndarray:
-        x = _f32(x)
+        x = _batch(x, self.cfg.n_in, "x")
         B = x.shape[0]
         y = np.empty((B, self.cfg.n_out), dtype=np.float32)
         for s in range(0, B, self.cfg.max_batch):
@@ -131,14 +141,18 @@ class MLP:
         return y
 
     def loss_grad(self, x, t):
-        x, t = _f32(x), _f32(t)
+        x, t = _batch(x, self.cfg.n_in, "x"), _batch(t, self.cfg.n_out, "t")
+        if x.shape[0] != t.shape[0] or x.shape[0] > self.cfg.max_batch:
+            raise ValueError("x and t need the same batch size, at most max_batch")
         loss = ctypes.c_float()
         g = np.empty(self.n_params, dtype=np.float32)
         self._check(_lib().ekpnp_mlp_loss_grad(self._h, _fp(x), _fp(t), x.shape[0], ctypes.byref(loss), _fp(g)))
         return float(loss.value), g
 
     def train_step(self, x, t, lr: float) -> float:
-        x, t = _f32(x), _f32(t)
+        x, t = _batch(x, self.cfg.n_in, "x"), _batch(t, self.cfg.n_out, "t")
+        if x.shape[0] != t.shape[0] or x.shape[0] > self.cfg.max_batch:
+            raise ValueError("x and t need the same batch size, at most max_batch")
         loss = ctypes.c_float()
         self._check(_lib().ekpnp_mlp_train_step(self._h, _fp(x), _fp(t), x.shape[0], lr, ctypes.byref(loss)))
         return float(loss.value)
@@ -151,12 +165,19 @@ class MLP:
 # features, targets, resampling
 # ---------------------------------------------------------------------------
 def features(zeta, kappa_h, e_x, ny) -> np.ndarray:
-    """[n, 4] network inputs, each roughly in [-1, 1]."""
+    """[n, 6] network inputs, each roughly in [-1, 1], all functions of
+    (zeta, kappa h, E_x, n_y): zeta, log kappa h, log E_x, log n_y, the Debye
+    resolution kappa dy = kappa h / ((n_y - 1) / 2) (the steady state's
+    departure from Poisson-Boltzmann is a wall-discretisation effect), and
+    the Gouy-Chapman charge scale sinh(|zeta| / 2)."""
     zeta, kappa_h, e_x, ny = (np.atleast_1d(np.asarray(v, dtype=np.float64)) for v in (zeta, kappa_h, e_x, ny))
+    kdy = kappa_h / (0.5 * (ny - 1.0))
     return np.stack([zeta / 2.0 + 1.0,
                      np.log(kappa_h) / np.log(50.0) * 2.0 - 1.0,
                      (np.log10(e_x) + 4.0) / 1.0,
-                     ny / 512.0 - 1.0], axis=1).astype(np.float32)
+                     np.log2(ny / 128.0) / 2.0,
+                     kdy / 0.2 - 1.0,
+                     np.sinh(0.5 * np.abs(zeta)) / np.sinh(2.0) * 2.0 - 1.0], axis=1).astype(np.float32)
 
 
 def resample(profile: np.ndarray, n: int) -> np.ndarray:
@@ -266,6 +287,22 @@ def baseline(p: ChannelParams):
     wall = lo + (hi - lo) * np.clip((y - 0.5) / max(p.ny - 2.0, 1.0), 0.0, 1.0)
     ux = hs_scale(p) * (psi - wall) / (-p.psi_lo)
     return psi, cpos, cneg, ux
+
+
+def osmotic_rho(p: ChannelParams, psi, cpos, cneg) -> np.ndarray:
+    """Density profile of the steady x-uniform channel implied by psi and
+    c+-: the wall-normal momentum balance cs^2 d(rho)/dy = F_y with the
+    body force F_y = force_coef (c+ - c-) E_y and E_y = -dpsi/dy by central
+    differences (copied to the wall rows, poisson.cu:161-172), integrated
+    with the trapezoidal rule and shifted to mean rho0 (the mass is
+    conserved)."""
+    psi = np.asarray(psi, dtype=np.float64)
+    ey = np.empty_like(psi)
+    ey[1:-1] = (psi[:-2] - psi[2:]) / (2.0 * p.dy)
+    ey[0], ey[-1] = ey[1], ey[-2]
+    fy = p.force_coef * (np.asarray(cpos) - np.asarray(cneg)) * ey
+    rho = np.concatenate([[0.0], np.cumsum(0.5 * (fy[1:] + fy[:-1]) * p.dy / p.cs2)])
+    return rho - rho.mean() + p.rho0
 
 
 def residual_targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:

@@ -14,10 +14,11 @@
 3. Held-out channels (another seed, nx from the config-2 set) are run three
    times: cold (uniform c0 + noise), warm (predicted profiles broadcast in x)
    and baseline-warm (the physics baseline alone).
-   Every warm channel first runs a probe of `probe` steps; a channel whose
-   residual after the probe exceeds `accept` (or is not finite) is rejected
+   Every warm channel first settles for 100 steps and then runs a probe of
+   `probe` steps; a channel whose u_x residual over the probe exceeds
+   `accept` (or is not finite) is rejected
    and restarted cold.  Both arms finish with the plain solver to the same
-   residual; step counts include the probe.
+   residual; step counts include settling and probe.
 
 Prints one JSON line with per-channel steps and the summary.
 """
@@ -121,8 +122,12 @@ def predictor(net, residual):
     return run
 
 
-def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None):
-    """Warm arm; corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
+def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None, settle=100):
+    """Warm arm.  Acceptance test: after `settle` steps (the local transient
+    of starting from equilibrium populations) the u_x residual over the next
+    `probe` steps must be <= accept and finite; a rejected channel restarts
+    from its cold fields.  Step counts include settle + probe.
+    corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
     t0 = time.perf_counter()
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
@@ -134,9 +139,11 @@ def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, a
             psi, cp, cn, ux = corrupt(c, psi, cp, cn, ux)
         ny, nx = p.ny, p.nx
         b = lambda v: np.repeat(v[:, None], nx, axis=1)  # noqa: E731
-        s.set_fields(c, np.full((ny, nx), p.rho0), b(ux), np.zeros((ny, nx)), b(cp), b(cn), b(psi))
+        rho = S.osmotic_rho(p, psi, cp, cn)
+        s.set_fields(c, b(rho), b(ux), np.zeros((ny, nx)), b(cp), b(cn), b(psi))
     s.init_equilibrium()
-    red0 = s.reductions()  # snapshot
+    s.step(settle)
+    s.reductions()  # residual snapshot
     s.step(probe)
     red = s.reductions()
     res = red[:, 7]
@@ -150,8 +157,7 @@ def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, a
     steps = run_to_residual(s, tol, every, max_steps, "warm")
     s.synchronize()
     wall = time.perf_counter() - t0
-    total = np.where(steps >= 0, steps + probe, -1)
-    del red0
+    total = np.where(steps >= 0, steps + settle + probe, -1)
     return s, total, wall, res, rejected
 
 
@@ -185,10 +191,14 @@ def main():
     ap.add_argument("--batch", type=int, default=128)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--probe", type=int, default=200)
-    ap.add_argument("--accept", type=float, default=1e-4)
+    ap.add_argument("--accept", type=float, default=5e-5,
+                    help="largest u_x residual after the probe for a warm start to be kept")
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--seed", type=int, default=1000)
-    ap.add_argument("--save", default="")
+    ap.add_argument("--save", default="", help="write the trained parameters and the training set (.npz)")
+    ap.add_argument("--load-data", default="",
+                    help="take the training targets and step counts from a --save file of the same "
+                         "--train/--seed/--ny/--nx-train/--targets instead of generating them")
     ap.add_argument("--targets", choices=("direct", "residual"), default="residual",
                     help="learn the profiles themselves or their difference from the physics baseline")
     args = ap.parse_args()
@@ -198,8 +208,15 @@ def main():
     tr_ch, rej_tr = S.sample_channels(args.train, args.seed, nx_choices=(args.nx_train,), ny_choices=ny)
     log(f"training channels: {len(tr_ch)} (redrawn {rej_tr})")
     residual = args.targets == "residual"
-    X, T, tr_steps, tr_wall = generate(tr_ch, args.precision, args.seed, args.tol, args.every, args.max_steps,
-                                       residual)
+    if args.load_data:
+        saved = np.load(args.load_data)
+        T, tr_steps, tr_wall = saved["T"], saved["steps"], 0.0
+        X = S.channel_features(tr_ch)
+        if T.shape[0] != len(tr_ch):
+            raise SystemExit("--load-data file holds another training set")
+    else:
+        X, T, tr_steps, tr_wall = generate(tr_ch, args.precision, args.seed, args.tol, args.every, args.max_steps,
+                                           residual)
     good = tr_steps > 0
     log(f"data: {good.sum()} converged, steps median {np.median(tr_steps[good]):.0f}, wall {tr_wall:.1f} s")
     rms = np.sqrt(np.mean(T[good].reshape(int(good.sum()), S.N_FIELDS, S.N_PTS) ** 2, axis=(0, 2)))
@@ -231,6 +248,7 @@ def main():
                     "cold_wall_s": round(cold_wall, 2), "warm_wall_s": round(warm_wall, 2),
                     "wall_ratio": round(cold_wall / warm_wall, 3), "rejected": rejected,
                     "probe_residual_median": float(np.median(probe_res)),
+                    "probe_residuals": [float(v) for v in probe_res],
                     "cold_steps": cold.tolist(), "warm_steps": warm.tolist(),
                     "summary": summarize(cold, warm),
                     "baseline_steps": base.tolist(), "baseline_rejected": base_rej,
@@ -239,7 +257,7 @@ def main():
         "targets": args.targets,
     }
     if args.save:
-        np.savez(args.save, params=net.get_params(), X=X, T=T)
+        np.savez(args.save, params=net.get_params(), X=X, T=T, steps=tr_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -54,11 +54,13 @@ def test_gradients_match_reference():
 
 
 def test_full_size_network_forward():
-    # the surrogate's own shape: 4 -> 4 x 256 GELU -> 1024
+    # the surrogate's own shape: N_IN -> 4 x 256 GELU -> 1024
     m = S.MLP(max_batch=256, seed=11)
-    shp = R.shapes(4, 256, 4, 1024)
+    shp = R.shapes(S.N_IN, 256, 4, 1024)
     p = m.get_params().astype(np.float64)
-    x = np.random.default_rng(3).uniform(-1, 1, (200, 4))
+    x = np.random.default_rng(3).uniform(-1, 1, (200, S.N_IN))
+    with pytest.raises(ValueError):
+        m.forward(x[:, :4])
     e = _rel(m.forward(x), R.forward(p, x, shp, emulate=True)[0])
     print("full forward rel", e, flush=True)
     assert e < 5e-3

@@ -68,3 +68,13 @@ def test_residual_targets_round_trip():
         assert np.abs(got - want).max() <= 5e-6 * np.abs(want).max()
     # the baseline itself maps to zero targets
     assert np.abs(S.residual_targets(p, psi, cp, cn, ux)).max() == 0.0
+
+
+def test_features_stay_in_range_over_the_sampling_ranges():
+    ps, _ = S.sample_channels(400, 5)
+    X = S.channel_features(ps)
+    assert X.shape == (400, S.N_IN) and X.dtype == np.float32
+    assert np.all(np.abs(X) <= 1.6), np.abs(X).max(axis=0)
+    # kappa dy feature is the Debye resolution the stability rule bounds
+    kdy = np.array([np.sqrt(2 * p.poisson_coef) * p.dy for p in ps])
+    assert np.allclose((X[:, 4] + 1.0) * 0.2, kdy, rtol=1e-5)
