@@ -292,17 +292,18 @@ def baseline(p: ChannelParams):
 def osmotic_rho(p: ChannelParams, psi, cpos, cneg) -> np.ndarray:
     """Density profile of the steady x-uniform channel implied by psi and
     c+-: the wall-normal momentum balance cs^2 d(rho)/dy = F_y with the
-    body force F_y = force_coef (c+ - c-) E_y and E_y = -dpsi/dy by central
-    differences (copied to the wall rows, poisson.cu:161-172), integrated
-    with the trapezoidal rule and shifted to mean rho0 (the mass is
-    conserved)."""
+    body force F_y = force_coef (c+ - c-) E_y, E_y = -dpsi/dy by central
+    differences, integrated with the trapezoidal rule over the interior rows
+    and shifted to mean rho0 there (the mass is conserved; the bounce-back
+    wall rows, whose density the balance does not describe, are set to
+    rho0)."""
     psi = np.asarray(psi, dtype=np.float64)
-    ey = np.empty_like(psi)
-    ey[1:-1] = (psi[:-2] - psi[2:]) / (2.0 * p.dy)
-    ey[0], ey[-1] = ey[1], ey[-2]
-    fy = p.force_coef * (np.asarray(cpos) - np.asarray(cneg)) * ey
-    rho = np.concatenate([[0.0], np.cumsum(0.5 * (fy[1:] + fy[:-1]) * p.dy / p.cs2)])
-    return rho - rho.mean() + p.rho0
+    ey = (psi[:-2] - psi[2:]) / (2.0 * p.dy)                     # rows 1 .. ny-2
+    fy = p.force_coef * (np.asarray(cpos)[1:-1] - np.asarray(cneg)[1:-1]) * ey
+    rho_i = np.concatenate([[0.0], np.cumsum(0.5 * (fy[1:] + fy[:-1]) * p.dy / p.cs2)])
+    rho = np.full(psi.shape, p.rho0)
+    rho[1:-1] = rho_i - rho_i.mean() + p.rho0
+    return rho
 
 
 def residual_targets(p: ChannelParams, psi, cpos, cneg, ux) -> np.ndarray:

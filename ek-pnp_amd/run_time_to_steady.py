@@ -179,19 +179,19 @@ def summarize(cold, warm):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=512)
+    ap.add_argument("--train", type=int, default=8192)
     ap.add_argument("--heldout", type=int, default=64)
-    ap.add_argument("--ny", default="64,128,192,256")
+    ap.add_argument("--ny", default="32,48,64,96,128")
     ap.add_argument("--nx-train", type=int, default=4)
-    ap.add_argument("--nx", default="32,64,128,256")
+    ap.add_argument("--nx", default="32,64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--every", type=int, default=100)
-    ap.add_argument("--max-steps", type=int, default=3_000_000)
-    ap.add_argument("--epochs", type=int, default=300)
-    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--max-steps", type=int, default=1_000_000)
+    ap.add_argument("--epochs", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--probe", type=int, default=200)
-    ap.add_argument("--accept", type=float, default=5e-5,
+    ap.add_argument("--accept", type=float, default=1e-4,
                     help="largest u_x residual after the probe for a warm start to be kept")
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--seed", type=int, default=1000)

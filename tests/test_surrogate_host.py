@@ -78,3 +78,21 @@ def test_features_stay_in_range_over_the_sampling_ranges():
     # kappa dy feature is the Debye resolution the stability rule bounds
     kdy = np.array([np.sqrt(2 * p.poisson_coef) * p.dy for p in ps])
     assert np.allclose((X[:, 4] + 1.0) * 0.2, kdy, rtol=1e-5)
+
+
+def test_osmotic_density_matches_the_oracle_steady_state():
+    # the warm start's rho(y) from the wall-normal momentum balance against
+    # the FP64 oracle's steady state (interior rows, up to the constant the
+    # unmodelled wall-row mass leaves, < 10 % of the profile's variation)
+    import oracle as O
+    import ekpnp
+    p = ekpnp.lattice_params(4, 32, tau=0.8, zeta=-2.0, kappa_h=6.0, e_x=1e-4)
+    o = O.Oracle(p)
+    o.initialization(ekpnp.COLD_NOISE, 3, 0)
+    o.init_equilibrium()
+    o.step(60000)
+    rho = o.get_field("rho").mean(axis=1)[1:-1]
+    pred = S.osmotic_rho(p, *(o.get_field(f).mean(axis=1) for f in ("psi", "cpos", "cneg")))[1:-1]
+    dev = rho - rho.mean()
+    assert np.linalg.norm((pred - pred.mean()) - dev) < 0.02 * np.linalg.norm(dev)
+    assert np.linalg.norm(pred - rho) < 0.1 * np.linalg.norm(dev)
