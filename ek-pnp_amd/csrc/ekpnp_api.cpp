@@ -261,7 +261,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
       int rows = 1;
       if (d.pow2) {
         const int N = d.nx / 2;
-        rows = std::max(1, std::min(M, fft_elems / N));
+        rows = std::max(1, std::min(std::min(M, FFT_MAX_ROWS), fft_elems / N));
         // rows are padded to whole 16-element groups (LDS swizzle)
         const int padded = (rows * N + 15) & ~15;
         lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(padded * sizeof(double2)));
@@ -317,10 +317,9 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   for (const auto& d : c->desc) {
     if (!d.pow2) continue;
     const int N = d.nx / 2;
-    // rows, their spectra (N + 1 values each, FFT_PF + 1 per lane) and one
-    // X[N] slot per row must fit the persistent kernels' registers and LDS
-    if (N * d.fft_rows > FFT_PERSIST_ELEMS || (N + 1) * d.fft_rows > (FFT_PF + 1) * FFT_WG || d.fft_rows > 512)
-      persist_ok = false;
+    // the rows of a tile must fit the persistent kernels' prefetch registers
+    // (FFT_PF values or their spectrum pairs per lane) and LDS
+    if (N * d.fft_rows > FFT_PERSIST_ELEMS || d.fft_rows > FFT_MAX_ROWS) persist_ok = false;
   }
   c->sb.fft_persist = persist_ok ? 1 : 0;
   c->sb.fft_grid = std::max(1, std::min(n_fft_pow2, 2 * c->n_cu));
@@ -696,6 +695,19 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   int n_cu = 256;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device) == hipSuccess && n_cu > 0)
     c->n_cu = n_cu;
+  // default plan (ekpnp_autotune refines it): wide rows (nx >= 4096) run
+  // fastest one or two per work-group in the one-tile kernels, narrower rows
+  // in the persistent pipelined kernels with up to 1024 values per tile
+  int max_n = 0;
+  for (const auto& d : c->desc)
+    if (d.pow2) max_n = std::max(max_n, d.nx / 2);
+  if (max_n >= 2048) {
+    c->fft_elems = 4096;
+    c->fft_persist = 0;
+  } else {
+    c->fft_elems = 1024;
+    c->fft_persist = 1;
+  }
   he = build_poisson_plan(c, c->fft_elems, c->tri_rows, c->fft_persist);
   if (he != hipSuccess) {
     g_create_error = std::string("Poisson plan: ") + hipGetErrorString(he);

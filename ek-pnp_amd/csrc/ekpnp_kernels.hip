@@ -751,6 +751,44 @@ __device__ __forceinline__ long long spec_idx(int ny, int y, int k) {
   return ((long long)(k >> 3) * ny + y) * 8 + (k & 7);
 }
 
+// e / rows for e < 2^20, rows <= 2^12: (e + 1/2) / rows is at least 1/(2 rows)
+// away from an integer, far above the float rounding of the product
+__device__ __forceinline__ int div_rows(int e, float inv_rows) { return (int)(((float)e + 0.5f) * inv_rows); }
+
+// dst[j] = src[j] for j < n <= FFT_TW_SLOTS * FFT_WG, 16-byte values, by LDS
+// DMA (global_load_lds_dwordx4: lane l of a wave writes M0 + 16 l, M0 = the
+// wave's base in dst).  Nothing passes through registers; the data is in LDS
+// after the next s_waitcnt vmcnt(0), which the compiler places before the
+// next work-group barrier.  dst must be 16-byte aligned.
+__device__ __forceinline__ void lds_dma_copy(double2* dst, const double2* __restrict__ src, int n) {
+  const int wave_base = threadIdx.x & ~63;
+#pragma unroll
+  for (int c = 0; c < FFT_TW_SLOTS; ++c) {
+    const int j = threadIdx.x + c * FFT_WG;
+    if (c * FFT_WG < n && j < n)
+      __builtin_amdgcn_global_load_lds(src + j,
+                                       (__attribute__((address_space(3))) void*)(dst + c * FFT_WG + wave_base),
+                                       16, 0, 0);
+  }
+}
+
+// dst[x] = src[x] for x < n by one work-group, 8 loads per lane in flight
+__device__ __forceinline__ void copy_row_batched(double* __restrict__ dst, const double* __restrict__ src, int n) {
+  for (int x0 = 0; x0 < n; x0 += 8 * FFT_WG) {
+    double v[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int x = x0 + threadIdx.x + c * FFT_WG;
+      v[c] = x < n ? src[x] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int x = x0 + threadIdx.x + c * FFT_WG;
+      if (x < n) dst[x] = v[c];
+    }
+  }
+}
+
 // right-hand side of the lifted wall-normal system, scaled by dy^2
 // (odd_extension, poisson.cu:229-241)
 __device__ __forceinline__ double rhs_at(const double* __restrict__ re, const double* __restrict__ Wl,
@@ -766,11 +804,15 @@ __device__ __forceinline__ double rhs_at(const double* __restrict__ re, const do
 // k_fft_fwd: rows of RHS -> spectrum [y][k], k = 0..nx/2
 // ---------------------------------------------------------------------------
 // TWL: stage the quadrant twiddle table in LDS (else read it from global)
+// Every global load of a tile is issued before the first one is consumed
+// (twiddles, rows, wall lifts: up to 16 + 8 + 16 values per lane in flight),
+// so the load phase costs one memory latency, not one per element.
 template <bool POW2, bool TWL>
-__global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base) {
+__global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* __restrict__ tiles,
+                                                   const ChanDesc* __restrict__ chans, int tile_base) {
   extern __shared__ double2 lds[];
-  const int4 t = b.fft_tiles[tile_base + blockIdx.x];
-  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int4 t = tiles[tile_base + blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
   if (!cd->active) return;
   const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
   const double* __restrict__ re = b.rho_e + cd->fld_off;
@@ -780,32 +822,55 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
   double2* __restrict__ spec = b.spec + cd->spec_off;
   if (POW2) {
     const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    double2* twl = lds + ((rows * N + 15) & ~15);  // quadrant twiddles after the (swizzled) rows
-    if (TWL)
-      for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
+    const int total = rows << lgN;  // <= FFT_MAX_ELEMS
+    const double pc = cd->pc_dy2;
+    double2* twl = lds + ((total + 15) & ~15);  // quadrant twiddles after the (swizzled) rows
+    if (TWL) lds_dma_copy(twl, tw, quarter);
+    // the tile's rows are contiguous in rho_e: element e = (r, n) holds
+    // x[2n], x[2n+1] of row y0 + r as one 16-byte value (rows start 16-byte aligned)
+    const double2* __restrict__ src = reinterpret_cast<const double2*>(re + (long long)y0 * nx);
+    double2 v[FFT_PER_LANE];
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE; ++c)
+      if (c * FFT_WG < total) v[c] = src[min((int)threadIdx.x + c * FFT_WG, total - 1)];
+    // wall lifts of rows 1 and ny-2 (odd_extension, poisson.cu:229-241)
+    const bool wlo = (y0 == 1), whi = (y0 + rows - 1 == cd->ny - 2);
+    double2 wv[FFT_PER_LANE];
+    if (wlo || whi) {
+#pragma unroll
+      for (int c = 0; c < FFT_PER_LANE; ++c) {
+        wv[c] = make_double2(0.0, 0.0);
+        const int e = threadIdx.x + c * FFT_WG;
+        if (c * FFT_WG < total && e < total) {
+          const int r = e >> lgN, n = e & (N - 1);
+          const bool lo = wlo && r == 0, hi = whi && r == rows - 1;
+          if (lo || hi) {
+            const double* w = Wl + (lo ? W_PSI_LO : W_PSI_HI) * ws + 2 * n;
+            wv[c] = make_double2(w[0], w[1]);
+          }
+        }
+      }
+    }
     const double2* twq = TWL ? twl : tw;
-    const double2* r2 = reinterpret_cast<const double2*>(re);
-    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-      const int r = e >> lgN, n = e & (N - 1), y = y0 + r;
-      // x[2n], x[2n+1] of the row as one 16-byte load (rows start 16-byte aligned)
-      const double2 q = r2[(((long long)y * nx) >> 1) + n];
-      double v0 = -cd->pc_dy2 * q.x, v1 = -cd->pc_dy2 * q.y;
-      if (y == 1) {
-        v0 -= Wl[W_PSI_LO * ws + 2 * n];
-        v1 -= Wl[W_PSI_LO * ws + 2 * n + 1];
+#pragma unroll
+    for (int c = 0; c < FFT_PER_LANE; ++c) {
+      const int e = threadIdx.x + c * FFT_WG;
+      if (c * FFT_WG < total && e < total) {
+        double v0 = -pc * v[c].x, v1 = -pc * v[c].y;
+        if (wlo || whi) {
+          v0 -= wv[c].x;
+          v1 -= wv[c].y;
+        }
+        lds[sw(e)] = make_double2(v0, v1);
       }
-      if (y == cd->ny - 2) {
-        v0 -= Wl[W_PSI_HI * ws + 2 * n];
-        v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
-      }
-      lds[sw(e)] = make_double2(v0, v1);
     }
     __syncthreads();
     stockham<-1>(lds, rows, N, lgN, twq, lgnx);
     // split the packed transform: X[k] = E[k] + W^k O[k]
     // rows vary fastest across lanes: 8 modes x `rows` rows are contiguous
+    const float inv_rows = 1.0f / (float)rows;
     for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
-      const int k = e / rows, r = e - k * rows;
+      const int k = div_rows(e, inv_rows), r = e - k * rows;
       const double2 zk = lds[sw((r << lgN) + (k & (N - 1)))];
       const double2 zc = cconj(lds[sw((r << lgN) + ((N - k) & (N - 1)))]);
       const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
@@ -835,78 +900,80 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, int tile_base
 
 // ---------------------------------------------------------------------------
 // Persistent, software-pipelined power-of-two x-FFTs.  A work-group walks the
-// tiles blockIdx.x, blockIdx.x + gridDim.x, ...; while it transforms tile i
-// in LDS, the loads of tile i+1 are in flight into registers (FFT_PF values
-// per lane).  Tiles hold rows * N <= FFT_PF * FFT_WG complex values.  The
+// tiles blockIdx.x, blockIdx.x + gridDim.x, ...; while it transforms and
+// stores tile i, the loads of tile i+1 are in flight into registers.  Tile
+// and channel descriptors come through restrict pointers, so they are scalar
+// loads and never make the wave wait on the prefetch (barriers here wait on
+// LDS only).  Tiles hold rows * N <= FFT_PERSIST_ELEMS complex values.  The
 // quadrant twiddle table is (re)loaded into LDS when the channel changes.
 // ---------------------------------------------------------------------------
 static_assert(FFT_PF * FFT_WG == FFT_PERSIST_ELEMS, "persistent FFT tile size");
+constexpr int FFT_PF_PAIRS = (FFT_PERSIST_ELEMS / 2 + FFT_MAX_ROWS) / FFT_WG;  // spectrum pairs per lane
 
-struct FwdPrefetch {
-  double2 v[FFT_PF];
-};
-
-__device__ __forceinline__ void fwd_prefetch(const StepBuffers& b, int t, FwdPrefetch& pf) {
-  const int4 tt = b.fft_tiles[t];
-  const ChanDesc* __restrict__ cd = b.chans + tt.x;
-  const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), rows = tt.z;
-  const double2* __restrict__ r2 = reinterpret_cast<const double2*>(b.rho_e + cd->fld_off);
-  const int last = rows * N - 1;
+__device__ __forceinline__ void fwd_prefetch(const StepBuffers& b, const int4* __restrict__ tiles,
+                                             const ChanDesc* __restrict__ chans, int t, double2 (&pf)[FFT_PF]) {
+  const int4 tt = tiles[t];
+  const ChanDesc* __restrict__ cd = chans + tt.x;
+  const int nx = cd->nx, total = tt.z * (nx >> 1);
+  const double2* __restrict__ src = reinterpret_cast<const double2*>(b.rho_e + cd->fld_off + (long long)tt.y * nx);
 #pragma unroll
-  for (int c = 0; c < FFT_PF; ++c) {
-    const int e = min((int)threadIdx.x + c * FFT_WG, last);
-    const int r = e >> lgN, n = e & (N - 1);
-    pf.v[c] = r2[(((long long)(tt.y + r) * nx) >> 1) + n];
-  }
+  for (int c = 0; c < FFT_PF; ++c) pf[c] = src[min((int)threadIdx.x + c * FFT_WG, total - 1)];
 }
 
-__global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, int n_tiles) {
+__global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, const int4* __restrict__ tiles,
+                                                      const ChanDesc* __restrict__ chans, int n_tiles) {
   extern __shared__ double2 lds[];
-  const int max_elems = FFT_PF * FFT_WG;
-  double2* twl = lds + max_elems;  // quadrant twiddle table after the rows
+  double2* twl = lds + FFT_PERSIST_ELEMS;  // quadrant twiddle table after the rows
   int tw_chan = -1;
-  FwdPrefetch pf;
-  if ((int)blockIdx.x < n_tiles) fwd_prefetch(b, blockIdx.x, pf);
+  double2 pf[FFT_PF];
+  if ((int)blockIdx.x < n_tiles) fwd_prefetch(b, tiles, chans, blockIdx.x, pf);
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const int4 tt = b.fft_tiles[t];
-    const ChanDesc* __restrict__ cd = b.chans + tt.x;
+    const int4 tt = tiles[t];
+    const ChanDesc* __restrict__ cd = chans + tt.x;
     const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    const int y0 = tt.y, rows = tt.z;
+    const int y0 = tt.y, rows = tt.z, total = rows << lgN;
     const bool active = cd->active != 0;
     if (active) {
       if (tt.x != tw_chan) {
+        // through registers: an LDS-DMA copy here would make the compiler
+        // wait for the prefetch before every later LDS read
         const double2* __restrict__ tw = b.tw + cd->tw_off;
         for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
         tw_chan = tt.x;
       }
+      const double pc = cd->pc_dy2;
+      // wall lifts of rows 1 and ny-2 (odd_extension, poisson.cu:229-241);
+      // ny >= 4, so no element is in both
+      const bool wlo = (y0 == 1), whi = (y0 + rows - 1 == cd->ny - 2);
       const double* __restrict__ Wl = b.walls + cd->wall_off;
       const long long ws = b.walls_stride;
 #pragma unroll
       for (int c = 0; c < FFT_PF; ++c) {
         const int e = threadIdx.x + c * FFT_WG;
-        if (e < rows * N) {
-          const int r = e >> lgN, n = e & (N - 1), y = y0 + r;
-          double v0 = -cd->pc_dy2 * pf.v[c].x, v1 = -cd->pc_dy2 * pf.v[c].y;
-          if (y == 1) {
-            v0 -= Wl[W_PSI_LO * ws + 2 * n];
-            v1 -= Wl[W_PSI_LO * ws + 2 * n + 1];
-          }
-          if (y == cd->ny - 2) {
-            v0 -= Wl[W_PSI_HI * ws + 2 * n];
-            v1 -= Wl[W_PSI_HI * ws + 2 * n + 1];
+        if (e < total) {
+          double v0 = -pc * pf[c].x, v1 = -pc * pf[c].y;
+          if (wlo || whi) {
+            const int r = e >> lgN, n = e & (N - 1);
+            const bool lo = wlo && r == 0, hi = whi && r == rows - 1;
+            if (lo || hi) {
+              const double* w = Wl + (lo ? W_PSI_LO : W_PSI_HI) * ws + 2 * n;
+              v0 -= w[0];
+              v1 -= w[1];
+            }
           }
           lds[sw(e)] = make_double2(v0, v1);
         }
       }
     }
-    // the next tile's loads overlap this tile's transform
-    if (t + (int)gridDim.x < n_tiles) fwd_prefetch(b, t + gridDim.x, pf);
+    // the next tile's loads overlap this tile's transform and stores
+    if (t + (int)gridDim.x < n_tiles) fwd_prefetch(b, tiles, chans, t + gridDim.x, pf);
     if (active) {
       __syncthreads();
       stockham<-1>(lds, rows, N, lgN, twl, lgnx);
       double2* __restrict__ spec = b.spec + cd->spec_off;
+      const float inv_rows = 1.0f / (float)rows;
       for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
-        const int k = e / rows, r = e - k * rows;
+        const int k = div_rows(e, inv_rows), r = e - k * rows;
         const double2 zk = lds[sw((r << lgN) + (k & (N - 1)))];
         const double2 zc = cconj(lds[sw((r << lgN) + ((N - k) & (N - 1)))]);
         const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
@@ -919,168 +986,173 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, int n_ti
   }
 }
 
-constexpr int FFT_PF_INV = FFT_PF + 1;  // rows * (N + 1) spectrum values
-
-
-// e / rows for e < 2^20, rows <= 2^12: (e + 1/2) / rows is at least 1/(2 rows)
-// away from an integer, far above the float rounding of the product
-__device__ __forceinline__ int div_rows(int e, float inv_rows) { return (int)(((float)e + 0.5f) * inv_rows); }
-
-// spectrum index of prefetch slot c of tile t (slots past the tile repeat
-// its last value, so every lane loads every slot unconditionally)
-__device__ __forceinline__ long long inv_slot(const StepBuffers& b, const int4& tt, int c) {
-  const ChanDesc* __restrict__ cd = b.chans + tt.x;
-  const int N = cd->nx >> 1, rows = tt.z;
-  const int e = min((int)threadIdx.x + c * FFT_WG, rows * (N + 1) - 1);
-  const int k = div_rows(e, 1.0f / (float)rows), r = e - k * rows;
-  return cd->spec_off + spec_idx(cd->ny, tt.y + r, k);
+// spectrum pairs (X[p], X[N-p]) of tile t, p = e / rows, r = e - p rows
+__device__ __forceinline__ void inv_prefetch(const StepBuffers& b, const int4* __restrict__ tiles,
+                                             const ChanDesc* __restrict__ chans, int t,
+                                             double2 (&xa)[FFT_PF_PAIRS], double2 (&xb)[FFT_PF_PAIRS]) {
+  const int4 tt = tiles[t];
+  const ChanDesc* __restrict__ cd = chans + tt.x;
+  const int N = cd->nx >> 1, rows = tt.z, ny = cd->ny;
+  const int npair = rows * ((N >> 1) + 1);
+  const float inv_rows = 1.0f / (float)rows;
+  const double2* __restrict__ spec = b.spec + cd->spec_off;
+#pragma unroll
+  for (int c = 0; c < FFT_PF_PAIRS; ++c) {
+    const int e = min((int)threadIdx.x + c * FFT_WG, npair - 1);
+    const int p = div_rows(e, inv_rows), r = e - p * rows;
+    xa[c] = spec[spec_idx(ny, tt.y + r, p)];
+    xb[c] = spec[spec_idx(ny, tt.y + r, N - p)];
+  }
 }
 
-__global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, int n_tiles) {
+__global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, const int4* __restrict__ tiles,
+                                                      const ChanDesc* __restrict__ chans, int n_tiles) {
   extern __shared__ double2 lds[];
-  const int max_elems = FFT_PF * FFT_WG;
-  double2* twl = lds + max_elems;            // quadrant twiddle table (N/2 <= max_elems/2 values)
-  double2* side = twl + (max_elems >> 1);    // X[N] of each row of the tile
+  double2* twl = lds + FFT_PERSIST_ELEMS;  // quadrant twiddle table after the rows
   int tw_chan = -1;
-  double2 p0, p1, p2, p3, p4, p5, p6, p7, p8;
-#define EKPNP_INV_PREFETCH(T)                                  \
-  {                                                            \
-    const int4 tn = b.fft_tiles[T];                            \
-    p0 = b.spec[inv_slot(b, tn, 0)];                           \
-    p1 = b.spec[inv_slot(b, tn, 1)];                           \
-    p2 = b.spec[inv_slot(b, tn, 2)];                           \
-    p3 = b.spec[inv_slot(b, tn, 3)];                           \
-    p4 = b.spec[inv_slot(b, tn, 4)];                           \
-    p5 = b.spec[inv_slot(b, tn, 5)];                           \
-    p6 = b.spec[inv_slot(b, tn, 6)];                           \
-    p7 = b.spec[inv_slot(b, tn, 7)];                           \
-    p8 = b.spec[inv_slot(b, tn, 8)];                           \
-  }
-  if ((int)blockIdx.x < n_tiles) EKPNP_INV_PREFETCH(blockIdx.x)
+  double2 xa[FFT_PF_PAIRS], xb[FFT_PF_PAIRS];
+  if ((int)blockIdx.x < n_tiles) inv_prefetch(b, tiles, chans, blockIdx.x, xa, xb);
   for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const int4 tt = b.fft_tiles[t];
-    const ChanDesc* __restrict__ cd = b.chans + tt.x;
-    const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    const int y0 = tt.y, rows = tt.z;
+    const int4 tt = tiles[t];
+    const ChanDesc* __restrict__ cd = chans + tt.x;
+    const int nx = cd->nx, N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2, half = N >> 1;
+    const int y0 = tt.y, rows = tt.z, ny = cd->ny;
+    const int npair = rows * (half + 1);
     const bool active = cd->active != 0;
-    double* __restrict__ ps = b.psi + cd->fld_off;
     if (active) {
       if (tt.x != tw_chan) {
+        // through registers (see k_fft_fwd_p)
         const double2* __restrict__ tw = b.tw + cd->tw_off;
         for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
         tw_chan = tt.x;
+        __syncthreads();
       }
-      // Dirichlet rows psi(0), psi(ny-1) (odd_extract, poisson.cu:297-303),
-      // written by the tiles holding rows 1 and ny-2
-      const double* __restrict__ Wl = b.walls + cd->wall_off;
-      const long long ws = b.walls_stride;
-      if (y0 == 1)
-        for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[x] = Wl[W_PSI_LO * ws + x];
-      if (y0 + rows - 1 == cd->ny - 2)
-        for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[(long long)(cd->ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
-      // raw spectrum into the rows (X[N] of each row aside)
+      // z[k] = E + i O, E = (X[k] + conj X[N-k]) / 2,
+      // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
       const float inv_rows = 1.0f / (float)rows;
-      const double2 pv[FFT_PF_INV] = {p0, p1, p2, p3, p4, p5, p6, p7, p8};
 #pragma unroll
-      for (int c = 0; c < FFT_PF_INV; ++c) {
+      for (int c = 0; c < FFT_PF_PAIRS; ++c) {
         const int e = threadIdx.x + c * FFT_WG;
-        if (e < rows * (N + 1)) {
-          const int k = div_rows(e, inv_rows), r = e - k * rows;
-          if (k < N) lds[sw((r << lgN) + k)] = pv[c];
-          else side[r] = pv[c];
+        if (e < npair) {
+          const int p = div_rows(e, inv_rows), r = e - p * rows;
+          const int rb = r << lgN;
+          {
+            const double2 xk = xa[c], xc = cconj(xb[c]);
+            const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+            const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                                   cconj(tw_half(twl, p, quarter)));
+            lds[sw(rb + p)] = make_double2(E.x - O.y, E.y + O.x);
+          }
+          if (p > 0 && p < half) {
+            const double2 xk = xb[c], xc = cconj(xa[c]);
+            const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+            const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                                   cconj(tw_half(twl, N - p, quarter)));
+            lds[sw(rb + N - p)] = make_double2(E.x - O.y, E.y + O.x);
+          }
         }
       }
     }
-    if (t + (int)gridDim.x < n_tiles) EKPNP_INV_PREFETCH(t + gridDim.x)
+    if (t + (int)gridDim.x < n_tiles) inv_prefetch(b, tiles, chans, t + gridDim.x, xa, xb);
     if (active) {
       __syncthreads();
-      // merge pairs (k, N-k) in place: z[k] = E + i O,
-      // E = (X[k] + conj X[N-k]) / 2, O = (X[k] - conj X[N-k]) / 2 * conj W^k
-      for (int e = threadIdx.x; e < rows * (N / 2 + 1); e += FFT_WG) {
-        const int r = e / (N / 2 + 1), k = e - r * (N / 2 + 1);
-        const int rb = r << lgN;
-        const double2 a = lds[sw(rb + k)];
-        const double2 bb = (k == 0) ? side[r] : lds[sw(rb + N - k)];
-        const double2 xa = cconj(bb);
-        const double2 E = make_double2(0.5 * (a.x + xa.x), 0.5 * (a.y + xa.y));
-        const double2 O = cmul(make_double2(0.5 * (a.x - xa.x), 0.5 * (a.y - xa.y)), cconj(tw_half(twl, k, quarter)));
-        const double2 zk = make_double2(E.x - O.y, E.y + O.x);
-        if (k > 0 && k < N / 2) {
-          // partner N - k from (X[N-k], conj X[k])
-          const double2 xb = cconj(a);
-          const double2 E2 = make_double2(0.5 * (bb.x + xb.x), 0.5 * (bb.y + xb.y));
-          const double2 O2 = cmul(make_double2(0.5 * (bb.x - xb.x), 0.5 * (bb.y - xb.y)),
-                                  cconj(tw_half(twl, N - k, quarter)));
-          lds[sw(rb + N - k)] = make_double2(E2.x - O2.y, E2.y + O2.x);
-        }
-        lds[sw(rb + k)] = zk;
-      }
-      __syncthreads();
       stockham<+1>(lds, rows, N, lgN, twl, lgnx);
+      double* __restrict__ ps = b.psi + cd->fld_off;
       const double invN = 1.0 / (double)N;
+      double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)y0 * nx);
       for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-        const int r = e >> lgN, n = e & (N - 1);
         const double2 z = lds[sw(e)];
-        double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
-        *out = make_double2(z.x * invN, z.y * invN);
+        dst[e] = make_double2(z.x * invN, z.y * invN);
       }
-      __syncthreads();
+      // Dirichlet rows (odd_extract, poisson.cu:297-303)
+      const double* __restrict__ Wl = b.walls + cd->wall_off;
+      const long long ws = b.walls_stride;
+      if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
+      if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+      __syncthreads();  // LDS rows and twiddles are rewritten next
     }
   }
-#undef EKPNP_INV_PREFETCH
 }
 
 // ---------------------------------------------------------------------------
 // k_fft_inv: spectrum rows -> psi rows (normalised by 1/nx)
 // ---------------------------------------------------------------------------
+// Loads as in k_fft_fwd: all of a tile's spectrum values (as pairs X[p],
+// X[N-p]) and twiddles are in flight before the first is consumed.
 template <bool POW2, bool TWL>
-__global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, int tile_base) {
+__global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* __restrict__ tiles,
+                                                   const ChanDesc* __restrict__ chans, int tile_base) {
   extern __shared__ double2 lds[];
-  const int4 t = b.fft_tiles[tile_base + blockIdx.x];
-  const ChanDesc* __restrict__ cd = b.chans + t.x;
+  const int4 t = tiles[tile_base + blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
   if (!cd->active) return;
-  const int nx = cd->nx, nk = cd->nk, y0 = t.y, rows = t.z;
+  const int nx = cd->nx, nk = cd->nk, ny = cd->ny, y0 = t.y, rows = t.z;
   const double2* __restrict__ tw = b.tw + cd->tw_off;
   const double2* __restrict__ spec = b.spec + cd->spec_off;
   double* __restrict__ ps = b.psi + cd->fld_off;
-  // Dirichlet rows: psi(0) = voltage, psi(ny-1) = voltage2 (odd_extract,
-  // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
-  {
-    const double* __restrict__ Wl = b.walls + cd->wall_off;
-    const long long ws = b.walls_stride;
-    if (y0 == 1)
-      for (int x = threadIdx.x; x < nx; x += FFT_WG) ps[x] = Wl[W_PSI_LO * ws + x];
-    if (y0 + rows - 1 == cd->ny - 2)
-      for (int x = threadIdx.x; x < nx; x += FFT_WG)
-        ps[(long long)(cd->ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
-  }
   if (POW2) {
-    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
-    double2* twl = lds + ((rows * N + 15) & ~15);
-    if (TWL) {
-      for (int j = threadIdx.x; j < quarter; j += FFT_WG) twl[j] = tw[j];
-      __syncthreads();
-    }
+    const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2, half = N >> 1;
+    const int npair = rows * (half + 1);  // pairs (p, N - p), p = 0..N/2, of each row
+    double2* twl = lds + (((rows << lgN) + 15) & ~15);
+    if (TWL) lds_dma_copy(twl, tw, quarter);
+    // pair e: p = e / rows, r = e - p rows (rows vary fastest: the 8 modes of
+    // a spectrum block are one 128-byte line per row)
+    const float inv_rows = 1.0f / (float)rows;
+    double2 xa[FFT_PAIR_SLOTS], xb[FFT_PAIR_SLOTS];
+#pragma unroll
+    for (int c = 0; c < FFT_PAIR_SLOTS; ++c)
+      if (c * FFT_WG < npair) {
+        const int e = min((int)threadIdx.x + c * FFT_WG, npair - 1);
+        const int p = div_rows(e, inv_rows), r = e - p * rows;
+        xa[c] = spec[spec_idx(ny, y0 + r, p)];
+        xb[c] = spec[spec_idx(ny, y0 + r, N - p)];
+      }
+    if (TWL) __syncthreads();  // twiddles landed in LDS
     const double2* twq = TWL ? twl : tw;
-    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-      const int k = e / rows, r = e - k * rows;
-      const double2 xk = spec[spec_idx(cd->ny, y0 + r, k)];
-      const double2 xc = cconj(spec[spec_idx(cd->ny, y0 + r, N - k)]);
-      const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
-      const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)), cconj(tw_half(twq, k, quarter)));
-      lds[sw((r << lgN) + k)] = make_double2(E.x - O.y, E.y + O.x);  // E + i O
+    // z[k] = E + i O, E = (X[k] + conj X[N-k]) / 2,
+    // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
+#pragma unroll
+    for (int c = 0; c < FFT_PAIR_SLOTS; ++c) {
+      const int e = threadIdx.x + c * FFT_WG;
+      if (c * FFT_WG < npair && e < npair) {
+        const int p = div_rows(e, inv_rows), r = e - p * rows;
+        const int rb = r << lgN;
+        {
+          const double2 xk = xa[c], xc = cconj(xb[c]);
+          const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+          const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                                 cconj(tw_half(twq, p, quarter)));
+          lds[sw(rb + p)] = make_double2(E.x - O.y, E.y + O.x);
+        }
+        if (p > 0 && p < half) {
+          const double2 xk = xb[c], xc = cconj(xa[c]);
+          const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+          const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                                 cconj(tw_half(twq, N - p, quarter)));
+          lds[sw(rb + N - p)] = make_double2(E.x - O.y, E.y + O.x);
+        }
+      }
     }
     __syncthreads();
     stockham<+1>(lds, rows, N, lgN, twq, lgnx);
     const double invN = 1.0 / (double)N;
+    double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)y0 * nx);
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
-      const int r = e >> lgN, n = e & (N - 1);
       const double2 z = lds[sw(e)];
-      double2* out = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx) + n;
-      *out = make_double2(z.x * invN, z.y * invN);
+      dst[e] = make_double2(z.x * invN, z.y * invN);
     }
+    // Dirichlet rows psi(0) = voltage, psi(ny-1) = voltage2 (odd_extract,
+    // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
+    const double* __restrict__ Wl = b.walls + cd->wall_off;
+    const long long ws = b.walls_stride;
+    if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
+    if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
   } else {
-    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[spec_idx(cd->ny, y0, k)];
+    const double* __restrict__ Wl = b.walls + cd->wall_off;
+    const long long ws = b.walls_stride;
+    if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
+    if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[spec_idx(ny, y0, k)];
     __syncthreads();
     const int kmax = (nx - 1) / 2;  // modes counted twice
     const bool nyq = (nx & 1) == 0;
@@ -1459,31 +1531,33 @@ hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, co
 // then the direct-DFT tiles.
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_pow2 > 0 && b.fft_persist) {
-    hipLaunchKernelGGL(k_fft_fwd_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.n_fft_pow2);
+    hipLaunchKernelGGL(k_fft_fwd_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
+                       b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
-      hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+      hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
     else
-      hipLaunchKernelGGL((k_fft_fwd<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+      hipLaunchKernelGGL((k_fft_fwd<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
   }
   if (b.n_fft_tiles > b.n_fft_pow2)
     hipLaunchKernelGGL((k_fft_fwd<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
-                       b.fft_lds_dft, s, b, b.n_fft_pow2);
+                       b.fft_lds_dft, s, b, b.fft_tiles, b.chans, b.n_fft_pow2);
   return hipGetLastError();
 }
 
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_pow2 > 0 && b.fft_persist) {
-    hipLaunchKernelGGL(k_fft_inv_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.n_fft_pow2);
+    hipLaunchKernelGGL(k_fft_inv_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
+                       b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
-      hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+      hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
     else
-      hipLaunchKernelGGL((k_fft_inv<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, 0);
+      hipLaunchKernelGGL((k_fft_inv<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
   }
   if (b.n_fft_tiles > b.n_fft_pow2)
     hipLaunchKernelGGL((k_fft_inv<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
-                       b.fft_lds_dft, s, b, b.n_fft_pow2);
+                       b.fft_lds_dft, s, b, b.fft_tiles, b.chans, b.n_fft_pow2);
   return hipGetLastError();
 }
 
