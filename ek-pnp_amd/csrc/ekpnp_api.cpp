@@ -871,6 +871,42 @@ int ekpnp_fast_poisson(ekpnp_ctx* ctx) {
   return EKPNP_OK;
 }
 
+int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double tol, int max_iter, int check_every,
+                            int* iters_out, double* change_out) {
+  if (!ctx || !iters_out || !change_out) return EKPNP_ERR_INVALID;
+  if (channel < 0 || channel >= ctx->nchan) return fail(ctx, EKPNP_ERR_INVALID, "channel out of range");
+  if (!(omega > 0.0 && omega < 2.0)) return fail(ctx, EKPNP_ERR_INVALID, "omega must lie in (0, 2)");
+  if (!(tol >= 0.0) || max_iter < 1 || check_every < 1)
+    return fail(ctx, EKPNP_ERR_INVALID, "need tol >= 0, max_iter >= 1, check_every >= 1");
+  if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "no fields");
+  const ChanDesc& d = ctx->desc[channel];
+  if (d.nx & 1)
+    return fail(ctx, EKPNP_ERR_UNSUPPORTED, "the iterative Poisson solve needs an even nx (red-black colouring)");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ekpnp_channel_params& p = ctx->params[channel];
+  const double r = (p.dy / p.dx) * (p.dy / p.dx);
+  const double inv = 1.0 / (2.0 * (1.0 + r));
+  const int nwg = ((d.nx / 2) * (d.ny - 2) + LBM_WG - 1) / LBM_WG;  // per colour; 2 nwg <= LBM tiles
+  double* part = ctx->d_partials;
+  HIP_OR_FAIL(ctx, launch_psi_wall_rows(ctx->sb, channel, d.nx, ctx->stream));
+  int it = 0;
+  double change = INFINITY;
+  while (it < max_iter) {
+    HIP_OR_FAIL(ctx, launch_sor_sweep(ctx->sb, channel, d.nx, d.ny, 0, omega, r, inv, part, ctx->stream));
+    HIP_OR_FAIL(ctx, launch_sor_sweep(ctx->sb, channel, d.nx, d.ny, 1, omega, r, inv, part + nwg, ctx->stream));
+    ++it;
+    if (it % check_every == 0 || it == max_iter) {
+      HIP_OR_FAIL(ctx, launch_max_final(part, 2 * nwg, ctx->d_red, ctx->stream));
+      HIP_OR_FAIL(ctx, hipMemcpyAsync(&change, ctx->d_red, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+      HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+      if (change < tol) break;  // poisson.cu:57
+    }
+  }
+  *iters_out = it;
+  *change_out = change;
+  return EKPNP_OK;
+}
+
 int ekpnp_rebase(ekpnp_ctx* ctx) {
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");

@@ -123,5 +123,13 @@ hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s);
 hipError_t launch_rebase(int precision, const StepBuffers& b, void* cur, void* other, const double* new_refs,
                          hipStream_t s);
 hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s);
+// one red-black SOR half-sweep of channel chan's 5-point Poisson system
+// (color 0: x + y even); partials[wg] = largest |change| per work-group,
+// ceil((nx/2)(ny-2)/LBM_WG) of them
+hipError_t launch_sor_sweep(const StepBuffers& b, int chan, int nx, int ny, int color, double omega, double r,
+                            double inv, double* partials, hipStream_t s);
+hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s);
+// psi wall rows of channel chan from the wall potentials
+hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStream_t s);
 
 }  // namespace ekpnp

@@ -1500,6 +1500,86 @@ __global__ __launch_bounds__(LBM_WG) void k_pb_relax(StepBuffers b, double* __re
 }
 
 // ---------------------------------------------------------------------------
+// Iterative Poisson solve (the reference's poisson_phi / gpu_poisson,
+// poisson.cu:35-130): the 5-point finite-difference system
+//   (psi_L + psi_R - 2 psi)/dx^2 + (psi_U + psi_D - 2 psi)/dy^2 = -pc rho_e
+// on rows 1..ny-2, periodic in x, psi(0) and psi(ny-1) the wall potentials.
+// gpu_poisson relaxes all nodes in place at once (a racy mix of Jacobi and
+// Gauss-Seidel); one sweep here is a red-black SOR pass instead, the nodes
+// with (x + y) even (color 0) and then those with (x + y) odd, each
+//   psi_gs = (r (psi_L + psi_R) + psi_U + psi_D + pc dy^2 rho_e) * inv,
+//   r = (dy/dx)^2, inv = 1 / (2 (1 + r)),
+//   psi <- psi_gs (omega = 1) or psi + omega (psi_gs - psi),
+// which for omega = 1 and dx = dy is gpu_poisson's update (poisson.cu:122-124).
+// nx is even, so the colouring survives the periodic wrap.  Each work-group
+// writes the largest |change| of its nodes to partials[blockIdx.x].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+  return v;
+}
+
+__global__ __launch_bounds__(LBM_WG) void k_sor_sweep(StepBuffers b, int chan, int color, double omega, double r,
+                                                     double inv, double* __restrict__ partials) {
+  __shared__ double wmax[LBM_WG / 64];
+  const ChanDesc* __restrict__ cd = b.chans + chan;
+  const int nx = cd->nx, ny = cd->ny, hx = nx >> 1;
+  const int i = blockIdx.x * LBM_WG + (int)threadIdx.x;
+  double change = 0.0;
+  if (i < hx * (ny - 2)) {
+    const int y = 1 + i / hx;
+    const int x = 2 * (i - (y - 1) * hx) + ((y + color) & 1);
+    const int xm = x == 0 ? nx - 1 : x - 1, xp = x == nx - 1 ? 0 : x + 1;
+    double* __restrict__ ps = b.psi + cd->fld_off;
+    const double* __restrict__ row = ps + (long long)y * nx;
+    const double* __restrict__ Wl = b.walls + cd->wall_off;
+    const long long ws = b.walls_stride;
+    const double up = (y == ny - 2) ? Wl[W_PSI_HI * ws + x] : row[nx + x];
+    const double dn = (y == 1) ? Wl[W_PSI_LO * ws + x] : row[x - nx];
+    const double old = row[x];
+    const double src = cd->pc_dy2 * b.rho_e[cd->fld_off + (long long)y * nx + x];
+    const double gs = (r * (row[xm] + row[xp]) + up + dn + src) * inv;
+    const double nv = (omega == 1.0) ? gs : old + omega * (gs - old);
+    ps[(long long)y * nx + x] = nv;
+    change = fabs(nv - old);
+  }
+  change = wave_max(change);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = change;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = wmax[0];
+#pragma unroll
+    for (int w = 1; w < LBM_WG / 64; ++w) m = fmax(m, wmax[w]);
+    partials[blockIdx.x] = m;
+  }
+}
+
+// out[0] = max of v[0..n) (one work-group)
+__global__ __launch_bounds__(256) void k_max_final(const double* __restrict__ v, int n, double* __restrict__ out) {
+  __shared__ double wmax[4];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, v[i]);
+  m = wave_max(m);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) out[0] = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+}
+
+// psi(0) = W_PSI_LO, psi(ny-1) = W_PSI_HI of one channel (poisson.cu:96-103)
+__global__ __launch_bounds__(256) void k_psi_wall_rows(StepBuffers b, int chan) {
+  const ChanDesc* __restrict__ cd = b.chans + chan;
+  const int nx = cd->nx, ny = cd->ny;
+  const int x = blockIdx.x * 256 + (int)threadIdx.x;
+  if (x >= nx) return;
+  double* __restrict__ ps = b.psi + cd->fld_off;
+  const double* __restrict__ Wl = b.walls + cd->wall_off;
+  const long long ws = b.walls_stride;
+  ps[x] = Wl[W_PSI_LO * ws + x];
+  ps[(long long)(ny - 1) * nx + x] = Wl[W_PSI_HI * ws + x];
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename S, typename C>
@@ -1609,6 +1689,24 @@ hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s) {
 
 hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s) {
   hipLaunchKernelGGL(k_pb_relax, dim3(b.n_lbm_tiles), dim3(LBM_WG), 0, s, b, psi_old);
+  return hipGetLastError();
+}
+
+hipError_t launch_sor_sweep(const StepBuffers& b, int chan, int nx, int ny, int color, double omega, double r,
+                            double inv, double* partials, hipStream_t s) {
+  const int n = (nx / 2) * (ny - 2);
+  hipLaunchKernelGGL(k_sor_sweep, dim3((n + LBM_WG - 1) / LBM_WG), dim3(LBM_WG), 0, s, b, chan, color, omega, r, inv,
+                     partials);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, v, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStream_t s) {
+  hipLaunchKernelGGL(k_psi_wall_rows, dim3((nx + 255) / 256), dim3(256), 0, s, b, chan);
   return hipGetLastError();
 }
 

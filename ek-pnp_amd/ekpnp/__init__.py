@@ -94,6 +94,9 @@ SIGNATURES = [
     ("ekpnp_init_equilibrium", ctypes.c_int, [_VP]),
     ("ekpnp_stream_collide_save", ctypes.c_int, [_VP]),
     ("ekpnp_fast_poisson", ctypes.c_int, [_VP]),
+    ("ekpnp_poisson_iterative", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_double)]),
     ("ekpnp_step", ctypes.c_int, [_VP, ctypes.c_int]),
     ("ekpnp_rebase", ctypes.c_int, [_VP]),
     ("ekpnp_run_to_residual", ctypes.c_int, [_VP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -298,6 +301,18 @@ class Solver:
 
     def fast_poisson(self) -> None:
         self._check(lib().ekpnp_fast_poisson(self._ctx))
+
+    def poisson_iterative(self, channel: int = 0, omega: float = 1.0, tol: float = 1e-9,
+                          max_iter: int = 10 ** 6, check_every: int = 1) -> tuple:
+        """Red-black SOR solve of the 5-point Poisson system of one channel
+        (poisson_phi, poisson.cu:35-130; its defaults: tol 1e-9, 1e6 sweeps,
+        a convergence check every sweep).  Overwrites that channel's psi;
+        returns (sweeps, largest |change| of the last checked sweep)."""
+        it = ctypes.c_int(0)
+        ch = ctypes.c_double(0.0)
+        self._check(lib().ekpnp_poisson_iterative(self._ctx, int(channel), float(omega), float(tol), int(max_iter),
+                                                  int(check_every), ctypes.byref(it), ctypes.byref(ch)))
+        return it.value, ch.value
 
     def step(self, n: int = 1) -> None:
         self._check(lib().ekpnp_step(self._ctx, int(n)))

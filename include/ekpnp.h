@@ -204,6 +204,21 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx);
 /* ---- the coupled step ------------------------------------------------------ */
 int ekpnp_stream_collide_save(ekpnp_ctx* ctx);
 int ekpnp_fast_poisson(ekpnp_ctx* ctx);
+/* Iterative Poisson solve of one channel (replaces poisson_phi + gpu_poisson,
+ * poisson.cu:35-130): the 5-point finite-difference system in x and y,
+ * periodic in x, psi(0) and psi(ny-1) the wall potentials, right-hand side
+ * -pc (c+ - c-) from the current charge density.  Red-black SOR sweeps with
+ * relaxation omega in (0, 2) (omega = 1: Gauss-Seidel) starting from the
+ * current psi, which is overwritten.  Stops when the largest |change| of a
+ * sweep drops below tol (checked every check_every sweeps; the reference
+ * checks every sweep with tol 1e-9) or after max_iter sweeps (the reference
+ * uses 1e6, prints a message and goes on: here *change_out >= tol tells).
+ * A cross-check of ekpnp_fast_poisson, whose x-derivative is spectral: the
+ * two agree exactly for x-invariant charge and differ by the O(dx^2) x
+ * discretisation otherwise.  nx must be even (EKPNP_ERR_UNSUPPORTED).
+ * Synchronises every check_every sweeps. */
+int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double tol, int max_iter,
+                            int check_every, int* iters_out, double* change_out);
 /* With FP32 storage, ekpnp_step re-centres the storage reference every
  * EKPNP_REBASE_INTERVAL steps (see ekpnp_rebase). */
 int ekpnp_step(ekpnp_ctx* ctx, int nsteps);

@@ -184,3 +184,50 @@ class Oracle:
     @property
     def steps(self) -> int:
         return int(lib().ekpnp_oracle_steps(self._h))
+
+
+def poisson_sor(rho_e, psi, psi_lo, psi_hi, dx: float, dy: float, poisson_coef: float, omega: float = 1.0,
+                tol: float = 1e-9, max_iter: int = 10 ** 6, check_every: int = 1):
+    """Iterative 5-point Poisson solve: restatement of poisson_phi and
+    gpu_poisson (poisson.cu:35-130) in red-black order.
+
+    Solves (psi_L + psi_R - 2 psi)/dx^2 + (psi_U + psi_D - 2 psi)/dy^2
+    = -poisson_coef rho_e on rows 1..ny-2, periodic in x, with rows 0 and
+    ny-1 held at psi_lo(x), psi_hi(x) (poisson.cu:96-103).  gpu_poisson's
+    update psi = 0.25 (psi_L + psi_R + psi_U + psi_D + source), source =
+    charge/eps dx^2 (poisson.cu:118-124), is written for dx != dy as
+      psi_gs = (r (psi_L + psi_R) + psi_U + psi_D + poisson_coef dy^2 rho_e)
+               / (2 (1 + r)),  r = (dy/dx)^2,
+    and relaxed with omega (psi_gs itself for omega = 1).  The reference
+    updates every node at once, in place; here one sweep updates the nodes
+    with (x + y) even, then those with (x + y) odd (a Gauss-Seidel order with
+    the same fixed point).  Stops when the largest |change| of a sweep is
+    below tol (poisson.cu:50-57), tested every check_every sweeps, or after
+    max_iter sweeps.  Returns (psi, sweeps, change).
+    """
+    rho_e = np.asarray(rho_e, dtype=np.float64)
+    ny, nx = rho_e.shape
+    p = np.array(psi, dtype=np.float64, copy=True).reshape(ny, nx)
+    p[0] = psi_lo
+    p[ny - 1] = psi_hi
+    r = (dy / dx) ** 2
+    inv = 1.0 / (2.0 * (1.0 + r))
+    src = poisson_coef * dy * dy * rho_e
+    yy, xx = np.mgrid[0:ny, 0:nx]
+    interior = (yy >= 1) & (yy <= ny - 2)
+    masks = [interior & ((xx + yy) % 2 == c) for c in (0, 1)]
+    it, change = 0, np.inf
+    while it < max_iter:
+        ch = 0.0
+        for m in masks:
+            gs = (r * (np.roll(p, 1, axis=1) + np.roll(p, -1, axis=1)) + np.roll(p, -1, axis=0)
+                  + np.roll(p, 1, axis=0) + src) * inv
+            new = gs if omega == 1.0 else p + omega * (gs - p)
+            ch = max(ch, float(np.max(np.abs(new[m] - p[m]))))
+            p[m] = new[m]
+        it += 1
+        if it % check_every == 0 or it == max_iter:
+            change = ch
+            if change < tol:
+                break
+    return p, it, change
