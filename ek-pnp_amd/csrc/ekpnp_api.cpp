@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -29,6 +30,7 @@ const int CXH[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 // storage references: one record of 8 doubles per row, X (rho, c+, c-),
 // J (their x-fluxes), 2 unused
 constexpr int kRefStride = 8;
+constexpr int kMaxTriChunks = TRI_WG / 4;  // the plan keeps G = TRI_WG / C >= 4 modes per work-group
 
 // stored value + stored_ref(y, d) = population (FP32 storage; 0 for FP64)
 std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err);
@@ -69,6 +71,9 @@ struct ekpnp_ctx {
   void* lat[2] = {nullptr, nullptr};
   ChanDesc* d_desc = nullptr;
   int2* d_lbm_tiles = nullptr;
+  int2* d_lbm_split = nullptr;   // lbm_split on the device
+  std::vector<int2> lbm_split;   // LBM tiles for the step: row-kernel tiles, then the others
+  int n_lbm_row = 0;             // row-kernel tiles at the front of lbm_split
   int4* d_fft_tiles = nullptr;
   int2* d_tri_tiles = nullptr;
   int* d_first_tile = nullptr;
@@ -84,6 +89,8 @@ struct ekpnp_ctx {
   int2* d_row_tiles = nullptr;
   double* d_partials = nullptr;
   double* d_red = nullptr;
+  double* d_cprime = nullptr;    // chunk-entry Thomas coefficients (Poisson plan)
+  long long total_cprime = 0;    // its capacity: TRI_WG / 4 chunks x nk per channel
   size_t dev_bytes = 0;
   int max_lds = 65536;
   hipStream_t stream = nullptr;
@@ -212,6 +219,11 @@ StepBuffers make_buffers(ekpnp_ctx* c) {
   b.refs = c->d_refs;
   b.row_tiles = c->d_row_tiles;
   b.n_row_tiles = (int)c->row_tiles.size();
+  b.cprime = c->d_cprime;
+  b.lbm_row_tiles = c->d_lbm_split;
+  b.n_lbm_row_tiles = c->n_lbm_row;
+  b.lbm_gen_tiles = c->d_lbm_split + c->n_lbm_row;
+  b.n_lbm_gen_tiles = (int)c->lbm_split.size() - c->n_lbm_row;
   return b;
 }
 
@@ -230,7 +242,7 @@ void release(ekpnp_ctx* c) {
   void* ptrs[] = {c->lat[0], c->lat[1], c->d_desc, c->d_lbm_tiles, c->d_fft_tiles,
                   c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
                   c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red,
-                  c->d_refs, c->d_refs_new, c->d_row_tiles};
+                  c->d_refs, c->d_refs_new, c->d_row_tiles, c->d_cprime, c->d_lbm_split};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -282,7 +294,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
     ChanDesc& d = c->desc[i];
     const int M = d.ny - 2;
     int C = pow2_ceil((M + tri_rows - 1) / tri_rows);
-    C = std::max(8, std::min(128, C));
+    C = std::max(8, std::min(kMaxTriChunks, C));
     d.tri_C = C;
     d.tri_L = (M + C - 1) / C;
     d.tri_G = TRI_WG / C;
@@ -300,6 +312,11 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   if (e != hipSuccess) return e;
   e = hipMemcpy(c->d_desc, c->desc.data(), c->nchan * sizeof(ChanDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
+  // chunk-entry Thomas coefficients of the new chunking
+  for (int i = 0; i < c->nchan; ++i) {
+    e = launch_cprime_table(c->sb, i, c->desc[i].tri_C, c->desc[i].nk, c->stream);
+    if (e != hipSuccess) return e;
+  }
   c->sb.fft_tiles = c->d_fft_tiles;
   c->sb.n_fft_tiles = (int)c->fft_tiles.size();
   c->sb.n_fft_pow2 = n_fft_pow2;
@@ -562,7 +579,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   c->ctiles.resize(n_channels);
 
   // layout
-  long long pop = 0, fld = 0, spec = 0, refs = 0;
+  long long pop = 0, fld = 0, spec = 0, refs = 0, cprime = 0;
   int wall = 0;
   // every per-channel block starts on a 64-element boundary
   auto pad64 = [](long long v) { return (v + 63) / 64 * 64; };
@@ -574,12 +591,14 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     d.fld_off = fld;
     d.spec_off = spec;
     d.ref_off = refs;
+    d.cp_off = cprime;
     d.wall_off = wall;
     d.tw_off = wall;
     pop += 27 * d.plane;
     fld += pad64(nn);
     spec += pad64((long long)d.ny * d.nk_stride);
     refs += pad64((long long)kRefStride * d.ny);
+    cprime += pad64((long long)kMaxTriChunks * d.nk);
     wall += (int)pad64(d.nx);
     c->desc[i] = d;
   }
@@ -587,6 +606,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   c->total_nodes = fld;
   c->total_spec = spec;
   c->total_refs = refs;
+  c->total_cprime = cprime;
   c->total_nx = wall;
   for (int i = 0; i < n_channels; ++i)
     for (int y = 0; y < c->desc[i].ny; ++y) c->row_tiles.push_back(make_int2(i, y));
@@ -599,6 +619,22 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
+  // the step's split: interior-row tiles of channels whose rows are whole
+  // work-groups go to the row kernel (FP32 storage, FP64 arithmetic);
+  // EKPNP_LBM_ROWS=0 in the environment keeps every tile on k_lbm
+  {
+    const char* env = std::getenv("EKPNP_LBM_ROWS");
+    const bool use_rows = !(env && env[0] == '0');
+    std::vector<int2> gen;
+    for (const int2& t : c->lbm_tiles) {
+      const ChanDesc& d = c->desc[t.x];
+      const int y = t.y / d.nx;
+      if (use_rows && d.nx % LBM_WG == 0 && y >= 1 && y <= d.ny - 2) c->lbm_split.push_back(t);
+      else gen.push_back(t);
+    }
+    c->n_lbm_row = (int)c->lbm_split.size();
+    c->lbm_split.insert(c->lbm_split.end(), gen.begin(), gen.end());
+  }
   // Poisson tile tables: capacity for the finest decomposition (one row per
   // FFT tile, four modes per tridiagonal work-group)
   for (int i = 0; i < n_channels; ++i) {
@@ -635,6 +671,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     }
     if ((e = dalloc(c, &c->d_desc, n_channels)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_lbm_tiles, c->lbm_tiles.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_lbm_split, c->lbm_split.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_fft_tiles, c->fft_tiles_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_tri_tiles, c->tri_tiles_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_first_tile, c->first_tile.size())) != hipSuccess) return e;
@@ -650,6 +687,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = dalloc(c, &c->d_row_tiles, c->row_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_partials, 8 * c->lbm_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_red, 8 * (size_t)n_channels)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_cprime, c->total_cprime)) != hipSuccess) return e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return e;
     return hipSuccess;
   };
@@ -664,6 +702,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     hipError_t e;
     if ((e = hipMemcpy(c->d_desc, c->desc.data(), n_channels * sizeof(ChanDesc), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_lbm_tiles, c->lbm_tiles.data(), c->lbm_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
+    if ((e = hipMemcpy(c->d_lbm_split, c->lbm_split.data(), c->lbm_split.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_first_tile, c->first_tile.data(), c->first_tile.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_walls, c->walls.data(), c->walls.size() * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemcpy(c->d_tw, tw.data(), tw.size() * sizeof(double2), hipMemcpyHostToDevice)) != hipSuccess) return e;

@@ -45,6 +45,7 @@ struct ChanDesc {
   long long fld_off;            // first element of this channel's [ny][nx] fields
   long long spec_off;           // first double2 of this channel's spectrum blocks
   long long ref_off;            // first double of this channel's [ny][8] storage references
+  long long cp_off;             // first double of this channel's [tri_C][nk] chunk-entry Thomas coefficients
   int wall_off;                 // offset into each wall plane
   int tw_off;                   // offset of this channel's twiddles e^{-2 pi i j/nx}
   int pow2;                     // nx is a power of two (Stockham path)
@@ -71,6 +72,12 @@ struct StepBuffers {
   const ChanDesc* chans;
   const int2* lbm_tiles;        // (channel, first node) per LBM work-group
   int n_lbm_tiles;
+  // the same tiles split for the coupled step: interior rows of channels
+  // whose rows are whole work-groups (k_lbm_rows), then all the others (k_lbm)
+  const int2* lbm_row_tiles;
+  int n_lbm_row_tiles;
+  const int2* lbm_gen_tiles;
+  int n_lbm_gen_tiles;
   const int4* fft_tiles;        // (channel, first row, rows, 0) per FFT work-group
   int n_fft_tiles;
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
@@ -98,6 +105,7 @@ struct StepBuffers {
   double* refs;                 // storage references, [ny][8] per channel (rho, c+, c-, their x-fluxes, 2 unused)
   const int2* row_tiles;        // (channel, row) per row-means work-group
   int n_row_tiles;
+  double* cprime;               // Thomas coefficient entering chunk j of mode k, [tri_C][nk] per channel
 };
 
 // ---- launchers (ekpnp_kernels.hip) ----------------------------------------
@@ -108,6 +116,8 @@ hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, co
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s);
 hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s);
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
+// chunk-entry Thomas coefficients of channel chan for its current tri_C, tri_L
+hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s);
 hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* dst, hipStream_t s);
 // pre-collision populations of the next step, written unshifted into dst
 hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, void* dst,

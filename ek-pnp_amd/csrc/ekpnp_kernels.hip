@@ -308,10 +308,10 @@ __device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __rest
 // ---------------------------------------------------------------------------
 // S: storage type of the populations; C: arithmetic type of the collision.
 template <typename S, typename C, bool FROM_Q, bool MACRO>
-__global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, const S* __restrict__ src,
-                                                S* __restrict__ dst) {
+__global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, const int2* __restrict__ tiles,
+                                                const S* __restrict__ src, S* __restrict__ dst) {
   const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 tile = b.lbm_tiles[wg];
+  const int2 tile = tiles[wg];
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
   if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
@@ -427,6 +427,160 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
     const C j = C(rr.J[s] * tc);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop(Q + (9 * s + d) * plane + node, (S)to_stored(q[9 * s + d], d, j));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_lbm_rows: k_lbm for FP32 storage with FP64 arithmetic on the interior
+// rows (1 <= y <= ny-2) of channels whose rows are whole work-groups (nx a
+// multiple of LBM_WG), after the pre-collision (P') state.  Same operations
+// in the same order as k_lbm, so the same results, with fewer registers: the
+// row index and the three row records are work-group uniform (scalar
+// registers), and the 27 gathered values stay FP32, each species being
+// widened to its 9 FP64 rest deviations only while its moments or its
+// collision are computed (and stored right after its collision).
+// ---------------------------------------------------------------------------
+// FP64 rest deviations of species sp from its gathered FP32 values, as
+// load_pre computes them for an interior row
+__device__ __forceinline__ void widen_species(const float* v, const Refs3& rf, int sp, double tc, double* q) {
+  const double own = rf.o.X[sp];
+  const double dm = rf.m.X[sp] - own, dp = rf.p.X[sp] - own;
+  const double j0 = rf.o.J[sp] * tc, jm = rf.m.J[sp] * tc, jp = rf.p.J[sp] * tc;
+#pragma unroll
+  for (int d = 0; d < 9; ++d) q[d] = (double)v[d];
+  q[1] += WS * j0;
+  q[3] -= WS * j0;
+  q[2] += WS * dm;
+  q[5] += WD * (dm + jm);
+  q[6] += WD * (dm - jm);
+  q[4] += WS * dp;
+  q[7] += WD * (dp - jp);
+  q[8] += WD * (dp + jp);
+}
+
+template <bool MACRO>
+__global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
+                                                    const float* __restrict__ src, float* __restrict__ dst) {
+  using C = double;
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int2 tile = tiles[wg];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  if (!cd->active) return;  // converged channel, frozen by the residual driver
+  const int nx = cd->nx, ny = cd->ny;
+  const int y = tile.y / nx;  // the work-group's row, 1 <= y <= ny - 2
+  const int x = tile.y - y * nx + (int)threadIdx.x;
+  const long long node = (long long)y * nx + x;
+  const long long plane = cd->plane;
+  const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
+  float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
+  const double* __restrict__ ps = b.psi + cd->fld_off;
+  const Refs3 rf = load_refs3(b.refs + cd->ref_off, ny, y);
+  const RowRefs& rr = rf.o;
+  const double tc = cd->inv_cs2 * cd->inv_cfl;
+  const int xm = (x == 0) ? nx - 1 : x - 1;
+  const int xp = (x == nx - 1) ? 0 : x + 1;
+
+  // pull gather (LBM.cu:869-905; no y wrap on interior rows)
+  float v[27];
+  {
+    const long long r0 = (long long)y * nx, rm = r0 - nx, rp = r0 + nx;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+      const float* B = P + (long long)(9 * s) * plane;
+      v[9 * s + 0] = ld_pop(B + 0 * plane + r0 + x);
+      v[9 * s + 1] = ld_pop(B + 1 * plane + r0 + xm);
+      v[9 * s + 2] = ld_pop(B + 2 * plane + rm + x);
+      v[9 * s + 3] = ld_pop(B + 3 * plane + r0 + xp);
+      v[9 * s + 4] = ld_pop(B + 4 * plane + rp + x);
+      v[9 * s + 5] = ld_pop(B + 5 * plane + rm + xm);
+      v[9 * s + 6] = ld_pop(B + 6 * plane + rm + xp);
+      v[9 * s + 7] = ld_pop(B + 7 * plane + rp + xp);
+      v[9 * s + 8] = ld_pop(B + 8 * plane + rp + xm);
+    }
+  }
+  double Ex, Ey;
+  efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
+  // all 32 loads are in flight before the first use: keep the scheduler from
+  // interleaving the species' loads with the arithmetic
+  __builtin_amdgcn_sched_barrier(0);
+
+  // moments, species by species (the expressions of moments() and of the
+  // charge-density sums of k_lbm)
+  Moments<C> m;
+  C jx, jy, sp = 0.0, sn = 0.0;
+  {
+    C q[9];
+    widen_species(v, rf, 0, tc, q);
+    m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
+    jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
+    widen_species(v + 9, rf, 1, tc, q);
+    m.dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sp += q[k];
+    widen_species(v + 18, rf, 2, tc, q);
+    m.dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sn += q[k];
+  }
+  m.rho = C(rr.X[0]) + m.dr;
+  m.rhoe = (m.dcp - m.dcn) + C(rr.X[1] - rr.X[2]);
+  m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
+  m.Fy = C(cd->fcoef) * m.rhoe * C(Ey);
+  {
+    const C irho = C(1.0) / m.rho;
+    const C icfl = C(cd->inv_cfl), hdt = C(cd->half_dt);
+    m.ux = (jx * icfl + m.Fx * hdt) * irho;
+    m.uy = (jy * icfl + m.Fy * hdt) * irho;
+  }
+  const C ux = m.ux, uy = m.uy;
+  b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.X[1] - rr.X[2]);
+  if (MACRO) {
+    const long long o = cd->fld_off + node, st = b.macro_stride;
+    b.macro[M_RHO * st + o] = (double)m.rho;
+    b.macro[M_UX * st + o] = (double)ux;
+    b.macro[M_UY * st + o] = (double)uy;
+    b.macro[M_CP * st + o] = (double)(C(rr.X[1]) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(rr.X[2]) + m.dcn);
+  }
+
+  const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
+  // ions: drift velocity u + K E (LBM.cu:479-488), collide, store
+  {
+    const C Eix = C(Ex + cd->drift_app), Eiy = C(Ey);
+    const C kp = C(cd->kp), kn = C(cd->kn);
+    const C om_f = C(-0.5) * (ux * ux + uy * uy) * ics;
+    C q[9];
+    {
+      const C cp = C(rr.X[1]) + m.dcp;
+      const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
+      const C om_p = C(-0.5) * (vpx * vpx + vpy * vpy) * ics;
+      widen_species(v + 9, rf, 1, tc, q);
+      collide_ion(q, m.dcp, cp, vpx, vpy, C(cd->om_pp), C(cd->om_mp), ics, ticf, om_p);
+      const C j = C(rr.J[1] * tc);
+#pragma unroll
+      for (int d = 0; d < 9; ++d) st_pop(Q + (9 + d) * plane + node, (float)to_stored(q[d], d, j));
+    }
+    {
+      const C cn = C(rr.X[2]) + m.dcn;
+      const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
+      const C om_n = C(-0.5) * (vnx * vnx + vny * vny) * ics;
+      widen_species(v + 18, rf, 2, tc, q);
+      collide_ion(q, m.dcn, cn, vnx, vny, C(cd->om_pn), C(cd->om_mn), ics, ticf,
+                  (cd->flags & FLAG_G0_FLUID_U) ? om_f : om_n);
+      const C j = C(rr.J[2] * tc);
+#pragma unroll
+      for (int d = 0; d < 9; ++d) st_pop(Q + (18 + d) * plane + node, (float)to_stored(q[d], d, j));
+    }
+  }
+  {
+    C q[9];
+    widen_species(v, rf, 0, tc, q);
+    collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
+                  C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
+    const C j = C(rr.J[0] * tc);
+#pragma unroll
+    for (int d = 0; d < 9; ++d) st_pop(Q + d * plane + node, (float)to_stored(q[d], d, j));
   }
 }
 
@@ -1275,8 +1429,9 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
     yr[ii] = v.x;
     yi[ii] = v.y;
   }
-  // local forward sweep with y_{i0-1} = 0
-  double cprev = (i0 > 1 && len > 0) ? cprime_closed(i0 - 1, kd) : 0.0;
+  // local forward sweep with y_{i0-1} = 0; the coefficient entering the
+  // chunk, cprime_closed(i0 - 1, kd), comes from the plan's table
+  double cprev = (valid && i0 > 1 && len > 0) ? b.cprime[cd->cp_off + (long long)j * nk + k] : 0.0;
   double A = 1.0, pr = 0.0, pim = 0.0;
 #pragma unroll
   for (int ii = 0; ii < LMAX; ++ii) {
@@ -1331,6 +1486,17 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
     for (int ii = 0; ii < LMAX; ++ii)
       if (ii < len) col[(long long)(i0 + ii) * ks] = make_double2(yr[ii], yi[ii]);
   }
+}
+
+// table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
+// j L is the last row of chunk j-1); row 0 of the table is unused
+__global__ __launch_bounds__(256) void k_cprime_table(StepBuffers b, int chan) {
+  const ChanDesc* __restrict__ cd = b.chans + chan;
+  const int nk = cd->nk, C = cd->tri_C, L = cd->tri_L;
+  const int t = blockIdx.x * 256 + (int)threadIdx.x;
+  if (t >= C * nk) return;
+  const int j = t / nk, k = t - j * nk;
+  b.cprime[cd->cp_off + t] = j == 0 ? 0.0 : cprime_closed(j * L, (double)k * cd->kdy_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -1585,15 +1751,30 @@ __global__ __launch_bounds__(256) void k_psi_wall_rows(StepBuffers b, int chan) 
 template <typename S, typename C>
 static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, const void* src,
                                void* dst, hipStream_t s) {
-  const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
   const S* sp = static_cast<const S*>(src);
   S* dp = static_cast<S*>(dst);
+  const dim3 block(LBM_WG);
   if (from_q) {
-    if (macro) hipLaunchKernelGGL((k_lbm<S, C, true, true>), grid, block, 0, s, b, sp, dp);
-    else hipLaunchKernelGGL((k_lbm<S, C, true, false>), grid, block, 0, s, b, sp, dp);
-  } else {
-    if (macro) hipLaunchKernelGGL((k_lbm<S, C, false, true>), grid, block, 0, s, b, sp, dp);
-    else hipLaunchKernelGGL((k_lbm<S, C, false, false>), grid, block, 0, s, b, sp, dp);
+    const dim3 grid(b.n_lbm_tiles);
+    if (macro) hipLaunchKernelGGL((k_lbm<S, C, true, true>), grid, block, 0, s, b, b.lbm_tiles, sp, dp);
+    else hipLaunchKernelGGL((k_lbm<S, C, true, false>), grid, block, 0, s, b, b.lbm_tiles, sp, dp);
+    return hipGetLastError();
+  }
+  // the row kernel exists for FP32 storage with FP64 arithmetic
+  const bool rows = std::is_same<S, float>::value && std::is_same<C, double>::value && b.n_lbm_row_tiles > 0;
+  const int2* gen = rows ? b.lbm_gen_tiles : b.lbm_tiles;
+  const int n_gen = rows ? b.n_lbm_gen_tiles : b.n_lbm_tiles;
+  if (n_gen > 0) {
+    const dim3 grid(n_gen);
+    if (macro) hipLaunchKernelGGL((k_lbm<S, C, false, true>), grid, block, 0, s, b, gen, sp, dp);
+    else hipLaunchKernelGGL((k_lbm<S, C, false, false>), grid, block, 0, s, b, gen, sp, dp);
+  }
+  if (rows) {
+    const dim3 grid(b.n_lbm_row_tiles);
+    const float* fs = reinterpret_cast<const float*>(src);
+    float* fd = reinterpret_cast<float*>(dst);
+    if (macro) hipLaunchKernelGGL((k_lbm_rows<true>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+    else hipLaunchKernelGGL((k_lbm_rows<false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
   }
   return hipGetLastError();
 }
@@ -1638,6 +1819,11 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_tiles > b.n_fft_pow2)
     hipLaunchKernelGGL((k_fft_inv<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
                        b.fft_lds_dft, s, b, b.fft_tiles, b.chans, b.n_fft_pow2);
+  return hipGetLastError();
+}
+
+hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s) {
+  hipLaunchKernelGGL(k_cprime_table, dim3((tri_C * nk + 255) / 256), dim3(256), 0, s, b, chan);
   return hipGetLastError();
 }
 

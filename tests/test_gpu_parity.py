@@ -187,6 +187,48 @@ def test_fp32_arithmetic_variant_is_bounded():
     assert errs["ux"] < 1e-3
 
 
+@pytest.mark.parametrize("nx,ny", [(256, 40), (512, 24)])
+def test_row_kernel_is_bit_identical_to_the_general_kernel(nx, ny, monkeypatch):
+    # interior rows of channels whose rows are whole work-groups run on
+    # k_lbm_rows (FP32 storage, FP64 arithmetic); EKPNP_LBM_ROWS=0 sends every
+    # tile to k_lbm.  Same operations in the same order: identical bits,
+    # also with the macroscopic output of the last step and across a rebase.
+    p = ekpnp.lattice_params(nx, ny, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4)
+    runs = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("EKPNP_LBM_ROWS", env)
+        s = ekpnp.Solver([p], precision="fp32")
+        s.initialization(ekpnp.COLD_NOISE, 5)
+        s.init_equilibrium()
+        s.step(150)
+        s.rebase()
+        s.step(151)
+        runs.append({sp: s.get_populations(0, sp) for sp in ("f", "h", "g")} |
+                    {f: s.get_field(0, f) for f in ("rho", "ux", "cpos", "cneg", "psi")})
+    for k in runs[0]:
+        np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=k)
+
+
+def test_fp32_row_kernel_channel_against_the_oracle():
+    # a 256-wide FP32 channel (row kernel) and a 96-wide one (k_lbm) in one
+    # batch, each against its FP64 oracle after 2000 steps
+    pa = ekpnp.lattice_params(256, 34, tau=0.8, zeta=-1.0, kappa_h=6.0, e_x=1e-4)
+    pb = ekpnp.lattice_params(96, 40, tau=0.7, zeta=-1.5, kappa_h=5.0, e_x=2e-4)
+    s = ekpnp.Solver([pa, pb], precision="fp32")
+    s.initialization(ekpnp.COLD_NOISE, 7)
+    s.init_equilibrium()
+    s.step(2000)
+    for ch, p in enumerate((pa, pb)):
+        o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
+        o.initialization(ekpnp.COLD_NOISE, 7, ch)
+        o.init_equilibrium()
+        o.step(2000)
+        errs = field_errors(s, ch, o)
+        _report(f"fp32 batched ch={ch}", errs)
+        for f in ("ux", "cpos", "cneg", "psi"):
+            assert errs[f] < 1e-5, (ch, f, errs[f])
+
+
 def test_rebase_preserves_the_state():
     p = ekpnp.lattice_params(32, 48, tau=0.8, zeta=-2.0, kappa_h=8.0, e_x=1e-4)
     a = ekpnp.Solver([p], precision="fp32")
