@@ -98,7 +98,7 @@ def calibrate_copy():
 
 
 def measured_traffic(config, precision):
-    """HBM bytes per LBM stage (k_lbm_rows + k_lbm launches) from the committed counter summary of this
+    """HBM bytes per LBM stage (its interior- and wall-row launches) from the committed counter summary of this
     workload (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or
     None when no summary exists for it."""
     path = os.path.join(ROOT, "profiles", f"k_lbm_{config}_{precision}_traffic.json")
@@ -202,7 +202,7 @@ def main():
         "stages_ms": {k: round(v, 5) for k, v in st.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": "LBM stage: k_lbm_rows (interior rows) + k_lbm (wall rows), fused collide-stream",
+            "kernel": "LBM stage: k_lbm_rows<false,false> (interior rows) + k_lbm_rows<false,true> (wall rows), fused collide-stream",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

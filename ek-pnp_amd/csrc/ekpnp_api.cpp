@@ -72,8 +72,8 @@ struct ekpnp_ctx {
   ChanDesc* d_desc = nullptr;
   int2* d_lbm_tiles = nullptr;
   int2* d_lbm_split = nullptr;   // lbm_split on the device
-  std::vector<int2> lbm_split;   // LBM tiles for the step: row-kernel tiles, then the others
-  int n_lbm_row = 0;             // row-kernel tiles at the front of lbm_split
+  std::vector<int2> lbm_split;   // LBM tiles for the step: row-kernel interior, row-kernel wall, others
+  int n_lbm_row = 0, n_lbm_wall = 0;
   int4* d_fft_tiles = nullptr;
   int2* d_tri_tiles = nullptr;
   int* d_first_tile = nullptr;
@@ -222,8 +222,10 @@ StepBuffers make_buffers(ekpnp_ctx* c) {
   b.cprime = c->d_cprime;
   b.lbm_row_tiles = c->d_lbm_split;
   b.n_lbm_row_tiles = c->n_lbm_row;
-  b.lbm_gen_tiles = c->d_lbm_split + c->n_lbm_row;
-  b.n_lbm_gen_tiles = (int)c->lbm_split.size() - c->n_lbm_row;
+  b.lbm_wall_tiles = c->d_lbm_split + c->n_lbm_row;
+  b.n_lbm_wall_tiles = c->n_lbm_wall;
+  b.lbm_gen_tiles = c->d_lbm_split + c->n_lbm_row + c->n_lbm_wall;
+  b.n_lbm_gen_tiles = (int)c->lbm_split.size() - c->n_lbm_row - c->n_lbm_wall;
   return b;
 }
 
@@ -619,20 +621,23 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
-  // the step's split: interior-row tiles of channels whose rows are whole
-  // work-groups go to the row kernel (FP32 storage, FP64 arithmetic);
-  // EKPNP_LBM_ROWS=0 in the environment keeps every tile on k_lbm
+  // the step's split: tiles of channels whose rows are whole work-groups go
+  // to the row kernel (FP32 storage, FP64 arithmetic); EKPNP_LBM_ROWS=0 in
+  // the environment keeps every tile on k_lbm
   {
     const char* env = std::getenv("EKPNP_LBM_ROWS");
     const bool use_rows = !(env && env[0] == '0');
-    std::vector<int2> gen;
+    std::vector<int2> wall, gen;
     for (const int2& t : c->lbm_tiles) {
       const ChanDesc& d = c->desc[t.x];
       const int y = t.y / d.nx;
-      if (use_rows && d.nx % LBM_WG == 0 && y >= 1 && y <= d.ny - 2) c->lbm_split.push_back(t);
-      else gen.push_back(t);
+      if (!use_rows || d.nx % LBM_WG != 0) gen.push_back(t);
+      else if (y == 0 || y == d.ny - 1) wall.push_back(t);
+      else c->lbm_split.push_back(t);
     }
     c->n_lbm_row = (int)c->lbm_split.size();
+    c->n_lbm_wall = (int)wall.size();
+    c->lbm_split.insert(c->lbm_split.end(), wall.begin(), wall.end());
     c->lbm_split.insert(c->lbm_split.end(), gen.begin(), gen.end());
   }
   // Poisson tile tables: capacity for the finest decomposition (one row per

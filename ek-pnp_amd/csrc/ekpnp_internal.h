@@ -72,10 +72,13 @@ struct StepBuffers {
   const ChanDesc* chans;
   const int2* lbm_tiles;        // (channel, first node) per LBM work-group
   int n_lbm_tiles;
-  // the same tiles split for the coupled step: interior rows of channels
-  // whose rows are whole work-groups (k_lbm_rows), then all the others (k_lbm)
+  // the same tiles split for the coupled step: channels whose rows are whole
+  // work-groups go to k_lbm_rows (interior rows, then wall rows), all the
+  // others to k_lbm
   const int2* lbm_row_tiles;
   int n_lbm_row_tiles;
+  const int2* lbm_wall_tiles;
+  int n_lbm_wall_tiles;
   const int2* lbm_gen_tiles;
   int n_lbm_gen_tiles;
   const int4* fft_tiles;        // (channel, first row, rows, 0) per FFT work-group

@@ -431,17 +431,18 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
 }
 
 // ---------------------------------------------------------------------------
-// k_lbm_rows: k_lbm for FP32 storage with FP64 arithmetic on the interior
-// rows (1 <= y <= ny-2) of channels whose rows are whole work-groups (nx a
-// multiple of LBM_WG), after the pre-collision (P') state.  Same operations
-// in the same order as k_lbm, so the same results, with fewer registers: the
-// row index and the three row records are work-group uniform (scalar
-// registers), and the 27 gathered values stay FP32, each species being
-// widened to its 9 FP64 rest deviations only while its moments or its
-// collision are computed (and stored right after its collision).
+// k_lbm_rows: k_lbm for FP32 storage with FP64 arithmetic, for channels whose
+// rows are whole work-groups (nx a multiple of LBM_WG), after the
+// pre-collision (P') state.  Same operations in the same order as k_lbm, so
+// the same results, with fewer registers: the row index and the three row
+// records are work-group uniform (scalar registers), and the 27 values read
+// stay FP32, each species being widened to its 9 FP64 rest deviations only
+// while its moments or its collision are computed (and stored right after
+// its collision).  Wall rows take k_lbm's wall rules: the y = 0 velocity
+// mirror, the ions' anti-bounce-back of the node's own post-collision values
+// and the fluid's full-way bounce-back.
 // ---------------------------------------------------------------------------
-// FP64 rest deviations of species sp from its gathered FP32 values, as
-// load_pre computes them for an interior row
+// FP64 rest deviations of species sp from its gathered FP32 values (load_pre)
 __device__ __forceinline__ void widen_species(const float* v, const Refs3& rf, int sp, double tc, double* q) {
   const double own = rf.o.X[sp];
   const double dm = rf.m.X[sp] - own, dp = rf.p.X[sp] - own;
@@ -458,49 +459,118 @@ __device__ __forceinline__ void widen_species(const float* v, const Refs3& rf, i
   q[8] += WD * (dp + jp);
 }
 
-template <bool MACRO>
-__global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
-                                                    const float* __restrict__ src, float* __restrict__ dst) {
+// wall node, one ion species: anti-bounce-back of its own stored
+// post-collision values p (abb_own)
+__device__ __forceinline__ void abb_species(const float* p, double dcw, double j, double* q) {
+  const double two_s = (2.0 * WS) * dcw, two_d = (2.0 * WD) * dcw;
+  const double js = WS * j, jd = WD * j;
+  q[0] = -(double)p[0] + (2.0 * W0) * dcw;
+  q[3] = -((double)p[1] + js) + two_s;
+  q[4] = -(double)p[2] + two_s;
+  q[1] = -((double)p[3] - js) + two_s;
+  q[2] = -(double)p[4] + two_s;
+  q[7] = -((double)p[5] + jd) + two_d;
+  q[8] = -((double)p[6] - jd) + two_d;
+  q[5] = -((double)p[7] - jd) + two_d;
+  q[6] = -((double)p[8] + jd) + two_d;
+}
+
+// the 9 values of species s at row y (s = 0: gathered; ions: gathered, or on
+// a wall row anti-bounced from the node's own values, dcw = c_w - X_s)
+__device__ __forceinline__ void species_q(const float* v, const Refs3& rf, int s, double tc, bool wall, double dcw,
+                                          double* q) {
+  if (s > 0 && wall) abb_species(v, dcw, rf.o.J[s] * tc, q);
+  else widen_species(v, rf, s, tc, q);
+}
+
+// 9 FP32 values of species block B gathered for node x of row y (pull,
+// periodic in x and y; rm, r0, rp are the row offsets of y-1, y, y+1)
+__device__ __forceinline__ void gather9(const float* __restrict__ B, long long plane, long long rm, long long r0,
+                                        long long rp, int x, int xm, int xp, float* v) {
+  v[0] = ld_pop(B + 0 * plane + r0 + x);
+  v[1] = ld_pop(B + 1 * plane + r0 + xm);
+  v[2] = ld_pop(B + 2 * plane + rm + x);
+  v[3] = ld_pop(B + 3 * plane + r0 + xp);
+  v[4] = ld_pop(B + 4 * plane + rp + x);
+  v[5] = ld_pop(B + 5 * plane + rm + xm);
+  v[6] = ld_pop(B + 6 * plane + rm + xp);
+  v[7] = ld_pop(B + 7 * plane + rp + xp);
+  v[8] = ld_pop(B + 8 * plane + rp + xm);
+}
+
+// one node of a row tile (WALL: y = 0 or ny-1); a separate instantiation per
+// case keeps the interior code free of the wall rules' registers
+template <bool MACRO, bool WALL>
+__device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDesc* __restrict__ cd, int y, int x,
+                                             const float* __restrict__ P, float* __restrict__ Q) {
   using C = double;
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 tile = tiles[wg];
-  const ChanDesc* __restrict__ cd = b.chans + tile.x;
-  if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
-  const int y = tile.y / nx;  // the work-group's row, 1 <= y <= ny - 2
-  const int x = tile.y - y * nx + (int)threadIdx.x;
   const long long node = (long long)y * nx + x;
   const long long plane = cd->plane;
-  const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
-  float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
   const double* __restrict__ ps = b.psi + cd->fld_off;
-  const Refs3 rf = load_refs3(b.refs + cd->ref_off, ny, y);
+  const double* __restrict__ R = b.refs + cd->ref_off;
+  const Refs3 rf = load_refs3(R, ny, y);
   const RowRefs& rr = rf.o;
   const double tc = cd->inv_cs2 * cd->inv_cfl;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
+  const bool bottom = WALL && (y == 0), top = WALL && (y == ny - 1);
+  constexpr bool wall = WALL;
+  // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
+  // included (LBM.cu:402-443)
+  C mux = 0, muy = 0;
+  if (bottom) {
+    const Refs3 r1 = load_refs3(R, ny, 1);
+    float w[27];
+    const long long r0 = nx, rm = 0, rp = 2LL * nx;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, w + 9 * s);
+    double Ex1, Ey1;
+    efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
+    C q[9], dr, jx, jy, dcp, dcn;
+    widen_species(w, r1, 0, tc, q);
+    dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
+    jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
+    widen_species(w + 9, r1, 1, tc, q);
+    dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    widen_species(w + 18, r1, 2, tc, q);
+    dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    const C rho = C(r1.o.X[0]) + dr;
+    const C rhoe = (dcp - dcn) + C(r1.o.X[1] - r1.o.X[2]);
+    const C Fx = C(cd->fcoef) * rhoe * C(Ex1 + cd->e_app) + C(cd->exf);
+    const C Fy = C(cd->fcoef) * rhoe * C(Ey1);
+    const C irho = C(1.0) / rho;
+    const C icfl = C(cd->inv_cfl), hdt = C(cd->half_dt);
+    mux = -((jx * icfl + Fx * hdt) * irho);
+    muy = -((jy * icfl + Fy * hdt) * irho);
+  }
 
-  // pull gather (LBM.cu:869-905; no y wrap on interior rows)
+  // fluid: pull gather (LBM.cu:869-905, x and y wrap); ions: the same, or
+  // on a wall row the node's own values (LBM.cu:915-1023)
   float v[27];
   {
-    const long long r0 = (long long)y * nx, rm = r0 - nx, rp = r0 + nx;
+    const long long r0 = (long long)y * nx;
+    const long long rm = (long long)(bottom ? ny - 1 : y - 1) * nx, rp = (long long)(top ? 0 : y + 1) * nx;
+    gather9(P, plane, rm, r0, rp, x, xm, xp, v);
+    if (wall) {
 #pragma unroll
-    for (int s = 0; s < 3; ++s) {
-      const float* B = P + (long long)(9 * s) * plane;
-      v[9 * s + 0] = ld_pop(B + 0 * plane + r0 + x);
-      v[9 * s + 1] = ld_pop(B + 1 * plane + r0 + xm);
-      v[9 * s + 2] = ld_pop(B + 2 * plane + rm + x);
-      v[9 * s + 3] = ld_pop(B + 3 * plane + r0 + xp);
-      v[9 * s + 4] = ld_pop(B + 4 * plane + rp + x);
-      v[9 * s + 5] = ld_pop(B + 5 * plane + rm + xm);
-      v[9 * s + 6] = ld_pop(B + 6 * plane + rm + xp);
-      v[9 * s + 7] = ld_pop(B + 7 * plane + rp + xp);
-      v[9 * s + 8] = ld_pop(B + 8 * plane + rp + xm);
+      for (int k = 9; k < 27; ++k) v[k] = P[(long long)k * plane + node];
+    } else {
+#pragma unroll
+      for (int s = 1; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, v + 9 * s);
     }
+  }
+  double dcwp = 0.0, dcwn = 0.0;
+  if (wall) {
+    const double* __restrict__ Wl = b.walls + cd->wall_off + x;
+    const long long ws = b.walls_stride;
+    dcwp = (bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws]) - rr.X[1];
+    dcwn = (bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws]) - rr.X[2];
   }
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
-  // all 32 loads are in flight before the first use: keep the scheduler from
+  // every load is in flight before the first use: keep the scheduler from
   // interleaving the species' loads with the arithmetic
   __builtin_amdgcn_sched_barrier(0);
 
@@ -514,11 +584,11 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
     m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
     jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
     jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
-    widen_species(v + 9, rf, 1, tc, q);
+    species_q(v + 9, rf, 1, tc, wall, dcwp, q);
     m.dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
 #pragma unroll
     for (int k = 0; k < 9; ++k) sp += q[k];
-    widen_species(v + 18, rf, 2, tc, q);
+    species_q(v + 18, rf, 2, tc, wall, dcwn, q);
     m.dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
 #pragma unroll
     for (int k = 0; k < 9; ++k) sn += q[k];
@@ -533,7 +603,7 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
     m.ux = (jx * icfl + m.Fx * hdt) * irho;
     m.uy = (jy * icfl + m.Fy * hdt) * irho;
   }
-  const C ux = m.ux, uy = m.uy;
+  const C ux = bottom ? mux : m.ux, uy = bottom ? muy : m.uy;
   b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.X[1] - rr.X[2]);
   if (MACRO) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
@@ -555,7 +625,7 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
       const C cp = C(rr.X[1]) + m.dcp;
       const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
       const C om_p = C(-0.5) * (vpx * vpx + vpy * vpy) * ics;
-      widen_species(v + 9, rf, 1, tc, q);
+      species_q(v + 9, rf, 1, tc, wall, dcwp, q);
       collide_ion(q, m.dcp, cp, vpx, vpy, C(cd->om_pp), C(cd->om_mp), ics, ticf, om_p);
       const C j = C(rr.J[1] * tc);
 #pragma unroll
@@ -565,7 +635,7 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
       const C cn = C(rr.X[2]) + m.dcn;
       const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
       const C om_n = C(-0.5) * (vnx * vnx + vny * vny) * ics;
-      widen_species(v + 18, rf, 2, tc, q);
+      species_q(v + 18, rf, 2, tc, wall, dcwn, q);
       collide_ion(q, m.dcn, cn, vnx, vny, C(cd->om_pn), C(cd->om_mn), ics, ticf,
                   (cd->flags & FLAG_G0_FLUID_U) ? om_f : om_n);
       const C j = C(rr.J[2] * tc);
@@ -576,12 +646,45 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
   {
     C q[9];
     widen_species(v, rf, 0, tc, q);
-    collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
-                  C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
+    if (wall) {
+      // full-way bounce-back of the pre-collision state, f0 unchanged
+      // (LBM.cu:816-846); moving-wall terms on the top wall only
+      const C ms = top ? C(cd->multis) : C(0), md = top ? C(cd->multid) : C(0);
+      const C f1 = q[1], f2 = q[2], f3 = q[3], f4 = q[4];
+      const C f5 = q[5], f6 = q[6], f7 = q[7], f8 = q[8];
+      q[3] = f1 - ms;
+      q[4] = f2;
+      q[1] = f3 + ms;
+      q[2] = f4;
+      q[7] = f5 - md;
+      q[8] = f6 + md;
+      q[5] = f7 + md;
+      q[6] = f8 - md;
+    } else {
+      collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
+                    C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
+    }
     const C j = C(rr.J[0] * tc);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop(Q + d * plane + node, (float)to_stored(q[d], d, j));
   }
+}
+
+// WALL: the tiles are wall rows, else interior rows (two launches: one
+// kernel holding both paths would need the registers of both at once)
+template <bool MACRO, bool WALL>
+__global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
+                                                    const float* __restrict__ src, float* __restrict__ dst) {
+  const int wg = xcd_remap(blockIdx.x, gridDim.x);
+  const int2 tile = tiles[wg];
+  const ChanDesc* __restrict__ cd = b.chans + tile.x;
+  if (!cd->active) return;  // converged channel, frozen by the residual driver
+  const int nx = cd->nx;
+  const int y = tile.y / nx;  // the work-group's row
+  const int x = tile.y - y * nx + (int)threadIdx.x;
+  const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
+  float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
+  lbm_row_node<MACRO, WALL>(b, cd, y, x, P, Q);
 }
 
 // ---------------------------------------------------------------------------
@@ -1770,11 +1873,16 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, co
     else hipLaunchKernelGGL((k_lbm<S, C, false, false>), grid, block, 0, s, b, gen, sp, dp);
   }
   if (rows) {
-    const dim3 grid(b.n_lbm_row_tiles);
     const float* fs = reinterpret_cast<const float*>(src);
     float* fd = reinterpret_cast<float*>(dst);
-    if (macro) hipLaunchKernelGGL((k_lbm_rows<true>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
-    else hipLaunchKernelGGL((k_lbm_rows<false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+    if (b.n_lbm_wall_tiles > 0) {
+      const dim3 grid(b.n_lbm_wall_tiles);
+      if (macro) hipLaunchKernelGGL((k_lbm_rows<true, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
+      else hipLaunchKernelGGL((k_lbm_rows<false, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
+    }
+    const dim3 grid(b.n_lbm_row_tiles);
+    if (macro) hipLaunchKernelGGL((k_lbm_rows<true, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+    else hipLaunchKernelGGL((k_lbm_rows<false, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
   }
   return hipGetLastError();
 }
