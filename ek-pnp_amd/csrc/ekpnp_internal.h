@@ -15,13 +15,13 @@ enum WallPlane { W_PSI_LO = 0, W_PSI_HI = 1, W_CP_LO = 2, W_CP_HI = 3, W_CN_LO =
 enum MacroPlane { M_RHO = 0, M_UX = 1, M_UY = 2, M_CP = 3, M_CN = 4, M_COUNT = 5 };
 
 constexpr int LBM_WG = 256;       // nodes per LBM work-group (one node per lane)
-constexpr int FFT_WG = 256;       // lanes of an x-FFT work-group
+constexpr int FFT_WG = 256;       // lanes of a persistent x-FFT work-group
+constexpr int FFT_WG_ROWS = 512;  // lanes of a one-tile x-FFT work-group (radix-8 passes)
 constexpr int FFT_MAX_ELEMS = 4096;  // complex values held in LDS by one FFT work-group
-constexpr int FFT_PER_LANE = FFT_MAX_ELEMS / FFT_WG;  // 16 values per lane per FFT pass
+constexpr int FFT_ROW_SLOTS = FFT_MAX_ELEMS / FFT_WG_ROWS;  // values per lane of a one-tile load
 constexpr int FFT_MAX_ROWS = 256;  // rows per power-of-two FFT tile
-constexpr int FFT_TW_SLOTS = 8192 / 4 / FFT_WG;  // quadrant twiddles per lane (nx <= 8192)
 // spectrum pairs (X[p], X[N-p]) per lane: rows (N/2 + 1) <= FFT_MAX_ELEMS/2 + FFT_MAX_ROWS
-constexpr int FFT_PAIR_SLOTS = (FFT_MAX_ELEMS / 2 + FFT_MAX_ROWS) / FFT_WG;
+constexpr int FFT_PAIR_SLOTS = (FFT_MAX_ELEMS / 2 + FFT_MAX_ROWS + FFT_WG_ROWS - 1) / FFT_WG_ROWS;
 // persistent pipelined x-FFT: tiles of <= FFT_PERSIST_ELEMS complex values,
 // 8 prefetched per lane; LDS = rows + quadrant twiddles (nx <= 2 FFT_PERSIST_ELEMS)
 constexpr int FFT_PF = 8;
@@ -29,7 +29,6 @@ constexpr int FFT_PERSIST_ELEMS = FFT_PF * FFT_WG;  // 2048
 constexpr unsigned FFT_LDS_PERSIST = (FFT_PERSIST_ELEMS + FFT_PERSIST_ELEMS / 2) * 16;
 constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
-static_assert(FFT_TW_SLOTS * FFT_WG * 4 == MAX_NX, "twiddle slots cover nx <= MAX_NX");
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
 constexpr int TRI_LMAX = 16;              // rows per tridiagonal chunk (one lane)
 constexpr int MAX_INTERIOR_ROWS = 2048;  // 128 chunks x 16 rows per tridiagonal column

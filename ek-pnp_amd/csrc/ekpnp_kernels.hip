@@ -901,25 +901,26 @@ __device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
 
 // ---------------------------------------------------------------------------
 // Stockham autosort FFT of `rows` rows of length N = 2^lgN (4 <= N, and
-// rows * N <= 16 * FFT_WG) held in LDS, in place.  Stage (n, s) with radix R,
+// rows * N <= FFT_MAX_ELEMS) held in LDS, in place.  Stage (n, s) with radix R,
 // m = n / R: butterfly (p, q) reads x[q + s(p + j m)], j < R, takes the
 // R-point DFT in registers, multiplies output u by W_n^{p u} and writes
 // y[q + s(R p + u)]; a barrier separates the reads and the writes.  Radices:
 // one pass of 2^(lgN mod 4) when lgN is not a multiple of 4, then radix 16.
 // Each lane holds 16 values per pass.  twq is the quadrant table of e^{-2 pi i j/nx}.
 // ---------------------------------------------------------------------------
-template <int SIGN, int R>
+template <int SIGN, int R, int WGT>
 __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int lgn, int lgs,
                                               const double2* __restrict__ twq, int lgnx) {
   constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
-  constexpr int BPT = 16 / R;
+  // butterflies per lane: rows N / R <= FFT_MAX_ELEMS / R of them over WGT lanes
+  constexpr int BPT = (FFT_MAX_ELEMS / R + WGT - 1) / WGT;
   const int lgm = lgn - LGR, s = 1 << lgs;
   const int lgper = lgN - LGR;
   const int nb = rows << lgper;
   double2 v[BPT][R];
 #pragma unroll
   for (int c = 0; c < BPT; ++c) {
-    const int bf = threadIdx.x + c * FFT_WG;
+    const int bf = threadIdx.x + c * WGT;
     if (bf < nb) {
       const int r = bf >> lgper, jj = bf & ((1 << lgper) - 1);
       const int p = jj >> lgs, q = jj & (s - 1);
@@ -931,7 +932,7 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
   __syncthreads();
 #pragma unroll
   for (int c = 0; c < BPT; ++c) {
-    const int bf = threadIdx.x + c * FFT_WG;
+    const int bf = threadIdx.x + c * WGT;
     if (bf < nb) {
       const int r = bf >> lgper, jj = bf & ((1 << lgper) - 1);
       const int p = jj >> lgs, q = jj & (s - 1);
@@ -975,27 +976,30 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
   __syncthreads();
 }
 
-template <int SIGN>
+// WGT lanes; MAXR = 16 (16 values per lane per pass) or 8 (one more pass,
+// half the registers: the 512-lane one-tile kernels)
+template <int SIGN, int WGT, int MAXR>
 __device__ __forceinline__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ twq, int lgnx) {
+  constexpr int LGM = MAXR == 16 ? 4 : 3;
   int lgn = lgN, lgs = 0;
-  const int r = lgN & 3;
+  const int r = lgN % LGM;
   if (r == 1) {
-    stockham_pass<SIGN, 2>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 2, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 1;
     lgs += 1;
   } else if (r == 2) {
-    stockham_pass<SIGN, 4>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 4, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 2;
     lgs += 2;
   } else if (r == 3) {
-    stockham_pass<SIGN, 8>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 8, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 3;
     lgs += 3;
   }
-  while (lgn >= 4) {
-    stockham_pass<SIGN, 16>(x, rows, lgN, lgn, lgs, twq, lgnx);
-    lgn -= 4;
-    lgs += 4;
+  while (lgn >= LGM) {
+    stockham_pass<SIGN, MAXR, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    lgn -= LGM;
+    lgs += LGM;
   }
 }
 
@@ -1012,35 +1016,37 @@ __device__ __forceinline__ long long spec_idx(int ny, int y, int k) {
 // away from an integer, far above the float rounding of the product
 __device__ __forceinline__ int div_rows(int e, float inv_rows) { return (int)(((float)e + 0.5f) * inv_rows); }
 
-// dst[j] = src[j] for j < n <= FFT_TW_SLOTS * FFT_WG, 16-byte values, by LDS
+// dst[j] = src[j] for j < n <= MAX_NX / 4, 16-byte values, by LDS
 // DMA (global_load_lds_dwordx4: lane l of a wave writes M0 + 16 l, M0 = the
 // wave's base in dst).  Nothing passes through registers; the data is in LDS
 // after the next s_waitcnt vmcnt(0), which the compiler places before the
 // next work-group barrier.  dst must be 16-byte aligned.
+template <int WGT>
 __device__ __forceinline__ void lds_dma_copy(double2* dst, const double2* __restrict__ src, int n) {
   const int wave_base = threadIdx.x & ~63;
 #pragma unroll
-  for (int c = 0; c < FFT_TW_SLOTS; ++c) {
-    const int j = threadIdx.x + c * FFT_WG;
-    if (c * FFT_WG < n && j < n)
+  for (int c = 0; c < MAX_NX / 4 / WGT; ++c) {
+    const int j = threadIdx.x + c * WGT;
+    if (c * WGT < n && j < n)
       __builtin_amdgcn_global_load_lds(src + j,
-                                       (__attribute__((address_space(3))) void*)(dst + c * FFT_WG + wave_base),
+                                       (__attribute__((address_space(3))) void*)(dst + c * WGT + wave_base),
                                        16, 0, 0);
   }
 }
 
-// dst[x] = src[x] for x < n by one work-group, 8 loads per lane in flight
+// dst[x] = src[x] for x < n by one work-group of WGT lanes, 8 loads per lane in flight
+template <int WGT>
 __device__ __forceinline__ void copy_row_batched(double* __restrict__ dst, const double* __restrict__ src, int n) {
-  for (int x0 = 0; x0 < n; x0 += 8 * FFT_WG) {
+  for (int x0 = 0; x0 < n; x0 += 8 * WGT) {
     double v[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const int x = x0 + threadIdx.x + c * FFT_WG;
+      const int x = x0 + threadIdx.x + c * WGT;
       v[c] = x < n ? src[x] : 0.0;
     }
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
-      const int x = x0 + threadIdx.x + c * FFT_WG;
+      const int x = x0 + threadIdx.x + c * WGT;
       if (x < n) dst[x] = v[c];
     }
   }
@@ -1065,7 +1071,7 @@ __device__ __forceinline__ double rhs_at(const double* __restrict__ re, const do
 // (twiddles, rows, wall lifts: up to 16 + 8 + 16 values per lane in flight),
 // so the load phase costs one memory latency, not one per element.
 template <bool POW2, bool TWL>
-__global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const int4* __restrict__ tiles,
                                                    const ChanDesc* __restrict__ chans, int tile_base) {
   extern __shared__ double2 lds[];
   const int4 t = tiles[tile_base + blockIdx.x];
@@ -1082,23 +1088,23 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* _
     const int total = rows << lgN;  // <= FFT_MAX_ELEMS
     const double pc = cd->pc_dy2;
     double2* twl = lds + ((total + 15) & ~15);  // quadrant twiddles after the (swizzled) rows
-    if (TWL) lds_dma_copy(twl, tw, quarter);
+    if (TWL) lds_dma_copy<FFT_WG_ROWS>(twl, tw, quarter);
     // the tile's rows are contiguous in rho_e: element e = (r, n) holds
     // x[2n], x[2n+1] of row y0 + r as one 16-byte value (rows start 16-byte aligned)
     const double2* __restrict__ src = reinterpret_cast<const double2*>(re + (long long)y0 * nx);
-    double2 v[FFT_PER_LANE];
+    double2 v[FFT_ROW_SLOTS];
 #pragma unroll
-    for (int c = 0; c < FFT_PER_LANE; ++c)
-      if (c * FFT_WG < total) v[c] = src[min((int)threadIdx.x + c * FFT_WG, total - 1)];
+    for (int c = 0; c < FFT_ROW_SLOTS; ++c)
+      if (c * FFT_WG_ROWS < total) v[c] = src[min((int)threadIdx.x + c * FFT_WG_ROWS, total - 1)];
     // wall lifts of rows 1 and ny-2 (odd_extension, poisson.cu:229-241)
     const bool wlo = (y0 == 1), whi = (y0 + rows - 1 == cd->ny - 2);
-    double2 wv[FFT_PER_LANE];
+    double2 wv[FFT_ROW_SLOTS];
     if (wlo || whi) {
 #pragma unroll
-      for (int c = 0; c < FFT_PER_LANE; ++c) {
+      for (int c = 0; c < FFT_ROW_SLOTS; ++c) {
         wv[c] = make_double2(0.0, 0.0);
-        const int e = threadIdx.x + c * FFT_WG;
-        if (c * FFT_WG < total && e < total) {
+        const int e = threadIdx.x + c * FFT_WG_ROWS;
+        if (c * FFT_WG_ROWS < total && e < total) {
           const int r = e >> lgN, n = e & (N - 1);
           const bool lo = wlo && r == 0, hi = whi && r == rows - 1;
           if (lo || hi) {
@@ -1110,9 +1116,9 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* _
     }
     const double2* twq = TWL ? twl : tw;
 #pragma unroll
-    for (int c = 0; c < FFT_PER_LANE; ++c) {
-      const int e = threadIdx.x + c * FFT_WG;
-      if (c * FFT_WG < total && e < total) {
+    for (int c = 0; c < FFT_ROW_SLOTS; ++c) {
+      const int e = threadIdx.x + c * FFT_WG_ROWS;
+      if (c * FFT_WG_ROWS < total && e < total) {
         double v0 = -pc * v[c].x, v1 = -pc * v[c].y;
         if (wlo || whi) {
           v0 -= wv[c].x;
@@ -1122,11 +1128,11 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* _
       }
     }
     __syncthreads();
-    stockham<-1>(lds, rows, N, lgN, twq, lgnx);
+    stockham<-1, FFT_WG_ROWS, 8>(lds, rows, N, lgN, twq, lgnx);
     // split the packed transform: X[k] = E[k] + W^k O[k]
     // rows vary fastest across lanes: 8 modes x `rows` rows are contiguous
     const float inv_rows = 1.0f / (float)rows;
-    for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
+    for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG_ROWS) {
       const int k = div_rows(e, inv_rows), r = e - k * rows;
       const double2 zk = lds[sw((r << lgN) + (k & (N - 1)))];
       const double2 zc = cconj(lds[sw((r << lgN) + ((N - k) & (N - 1)))]);
@@ -1138,9 +1144,9 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_fwd(StepBuffers b, const int4* _
   } else {
     // direct real DFT of one row
     double* v = reinterpret_cast<double*>(lds);
-    for (int n = threadIdx.x; n < nx; n += FFT_WG) v[n] = rhs_at(re, Wl, ws, cd, n, y0);
+    for (int n = threadIdx.x; n < nx; n += FFT_WG_ROWS) v[n] = rhs_at(re, Wl, ws, cd, n, y0);
     __syncthreads();
-    for (int k = threadIdx.x; k < nk; k += FFT_WG) {
+    for (int k = threadIdx.x; k < nk; k += FFT_WG_ROWS) {
       double2 acc = make_double2(0.0, 0.0);
       int idx = 0;
       for (int n = 0; n < nx; ++n) {
@@ -1226,7 +1232,7 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_fwd_p(StepBuffers b, const in
     if (t + (int)gridDim.x < n_tiles) fwd_prefetch(b, tiles, chans, t + gridDim.x, pf);
     if (active) {
       __syncthreads();
-      stockham<-1>(lds, rows, N, lgN, twl, lgnx);
+      stockham<-1, FFT_WG, 16>(lds, rows, N, lgN, twl, lgnx);
       double2* __restrict__ spec = b.spec + cd->spec_off;
       const float inv_rows = 1.0f / (float)rows;
       for (int e = threadIdx.x; e < rows * (N + 1); e += FFT_WG) {
@@ -1313,7 +1319,7 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, const in
     if (t + (int)gridDim.x < n_tiles) inv_prefetch(b, tiles, chans, t + gridDim.x, xa, xb);
     if (active) {
       __syncthreads();
-      stockham<+1>(lds, rows, N, lgN, twl, lgnx);
+      stockham<+1, FFT_WG, 16>(lds, rows, N, lgN, twl, lgnx);
       double* __restrict__ ps = b.psi + cd->fld_off;
       const double invN = 1.0 / (double)N;
       double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)y0 * nx);
@@ -1324,8 +1330,8 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, const in
       // Dirichlet rows (odd_extract, poisson.cu:297-303)
       const double* __restrict__ Wl = b.walls + cd->wall_off;
       const long long ws = b.walls_stride;
-      if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
-      if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+      if (y0 == 1) copy_row_batched<FFT_WG>(ps, Wl + W_PSI_LO * ws, nx);
+      if (y0 + rows - 1 == ny - 2) copy_row_batched<FFT_WG>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
       __syncthreads();  // LDS rows and twiddles are rewritten next
     }
   }
@@ -1337,7 +1343,7 @@ __global__ __launch_bounds__(FFT_WG, 2) void k_fft_inv_p(StepBuffers b, const in
 // Loads as in k_fft_fwd: all of a tile's spectrum values (as pairs X[p],
 // X[N-p]) and twiddles are in flight before the first is consumed.
 template <bool POW2, bool TWL>
-__global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* __restrict__ tiles,
+__global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv(StepBuffers b, const int4* __restrict__ tiles,
                                                    const ChanDesc* __restrict__ chans, int tile_base) {
   extern __shared__ double2 lds[];
   const int4 t = tiles[tile_base + blockIdx.x];
@@ -1351,15 +1357,15 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* _
     const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2, half = N >> 1;
     const int npair = rows * (half + 1);  // pairs (p, N - p), p = 0..N/2, of each row
     double2* twl = lds + (((rows << lgN) + 15) & ~15);
-    if (TWL) lds_dma_copy(twl, tw, quarter);
+    if (TWL) lds_dma_copy<FFT_WG_ROWS>(twl, tw, quarter);
     // pair e: p = e / rows, r = e - p rows (rows vary fastest: the 8 modes of
     // a spectrum block are one 128-byte line per row)
     const float inv_rows = 1.0f / (float)rows;
     double2 xa[FFT_PAIR_SLOTS], xb[FFT_PAIR_SLOTS];
 #pragma unroll
     for (int c = 0; c < FFT_PAIR_SLOTS; ++c)
-      if (c * FFT_WG < npair) {
-        const int e = min((int)threadIdx.x + c * FFT_WG, npair - 1);
+      if (c * FFT_WG_ROWS < npair) {
+        const int e = min((int)threadIdx.x + c * FFT_WG_ROWS, npair - 1);
         const int p = div_rows(e, inv_rows), r = e - p * rows;
         xa[c] = spec[spec_idx(ny, y0 + r, p)];
         xb[c] = spec[spec_idx(ny, y0 + r, N - p)];
@@ -1370,8 +1376,8 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* _
     // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
 #pragma unroll
     for (int c = 0; c < FFT_PAIR_SLOTS; ++c) {
-      const int e = threadIdx.x + c * FFT_WG;
-      if (c * FFT_WG < npair && e < npair) {
+      const int e = threadIdx.x + c * FFT_WG_ROWS;
+      if (c * FFT_WG_ROWS < npair && e < npair) {
         const int p = div_rows(e, inv_rows), r = e - p * rows;
         const int rb = r << lgN;
         {
@@ -1391,10 +1397,10 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* _
       }
     }
     __syncthreads();
-    stockham<+1>(lds, rows, N, lgN, twq, lgnx);
+    stockham<+1, FFT_WG_ROWS, 8>(lds, rows, N, lgN, twq, lgnx);
     const double invN = 1.0 / (double)N;
     double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)y0 * nx);
-    for (int e = threadIdx.x; e < rows * N; e += FFT_WG) {
+    for (int e = threadIdx.x; e < rows * N; e += FFT_WG_ROWS) {
       const double2 z = lds[sw(e)];
       dst[e] = make_double2(z.x * invN, z.y * invN);
     }
@@ -1402,18 +1408,18 @@ __global__ __launch_bounds__(FFT_WG) void k_fft_inv(StepBuffers b, const int4* _
     // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
     const double* __restrict__ Wl = b.walls + cd->wall_off;
     const long long ws = b.walls_stride;
-    if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
-    if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+    if (y0 == 1) copy_row_batched<FFT_WG_ROWS>(ps, Wl + W_PSI_LO * ws, nx);
+    if (y0 + rows - 1 == ny - 2) copy_row_batched<FFT_WG_ROWS>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
   } else {
     const double* __restrict__ Wl = b.walls + cd->wall_off;
     const long long ws = b.walls_stride;
-    if (y0 == 1) copy_row_batched(ps, Wl + W_PSI_LO * ws, nx);
-    if (y0 + rows - 1 == ny - 2) copy_row_batched(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
-    for (int k = threadIdx.x; k < nk; k += FFT_WG) lds[k] = spec[spec_idx(ny, y0, k)];
+    if (y0 == 1) copy_row_batched<FFT_WG_ROWS>(ps, Wl + W_PSI_LO * ws, nx);
+    if (y0 + rows - 1 == ny - 2) copy_row_batched<FFT_WG_ROWS>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+    for (int k = threadIdx.x; k < nk; k += FFT_WG_ROWS) lds[k] = spec[spec_idx(ny, y0, k)];
     __syncthreads();
     const int kmax = (nx - 1) / 2;  // modes counted twice
     const bool nyq = (nx & 1) == 0;
-    for (int n = threadIdx.x; n < nx; n += FFT_WG) {
+    for (int n = threadIdx.x; n < nx; n += FFT_WG_ROWS) {
       double acc = lds[0].x;
       int idx = 0;
       for (int k = 1; k <= kmax; ++k) {
@@ -1904,12 +1910,12 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
                        b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
-      hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
+      hipLaunchKernelGGL((k_fft_fwd<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG_ROWS), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
     else
-      hipLaunchKernelGGL((k_fft_fwd<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
+      hipLaunchKernelGGL((k_fft_fwd<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG_ROWS), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
   }
   if (b.n_fft_tiles > b.n_fft_pow2)
-    hipLaunchKernelGGL((k_fft_fwd<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+    hipLaunchKernelGGL((k_fft_fwd<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG_ROWS),
                        b.fft_lds_dft, s, b, b.fft_tiles, b.chans, b.n_fft_pow2);
   return hipGetLastError();
 }
@@ -1920,12 +1926,12 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
                        b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {
     if (b.fft_tw_lds)
-      hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
+      hipLaunchKernelGGL((k_fft_inv<true, true>), dim3(b.n_fft_pow2), dim3(FFT_WG_ROWS), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
     else
-      hipLaunchKernelGGL((k_fft_inv<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
+      hipLaunchKernelGGL((k_fft_inv<true, false>), dim3(b.n_fft_pow2), dim3(FFT_WG_ROWS), b.fft_lds_pow2, s, b, b.fft_tiles, b.chans, 0);
   }
   if (b.n_fft_tiles > b.n_fft_pow2)
-    hipLaunchKernelGGL((k_fft_inv<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG),
+    hipLaunchKernelGGL((k_fft_inv<false, false>), dim3(b.n_fft_tiles - b.n_fft_pow2), dim3(FFT_WG_ROWS),
                        b.fft_lds_dft, s, b, b.fft_tiles, b.chans, b.n_fft_pow2);
   return hipGetLastError();
 }
