@@ -516,36 +516,6 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   const int xp = (x == nx - 1) ? 0 : x + 1;
   const bool bottom = WALL && (y == 0), top = WALL && (y == ny - 1);
   constexpr bool wall = WALL;
-  // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
-  // included (LBM.cu:402-443)
-  C mux = 0, muy = 0;
-  if (bottom) {
-    const Refs3 r1 = load_refs3(R, ny, 1);
-    float w[27];
-    const long long r0 = nx, rm = 0, rp = 2LL * nx;
-#pragma unroll
-    for (int s = 0; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, w + 9 * s);
-    double Ex1, Ey1;
-    efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
-    C q[9], dr, jx, jy, dcp, dcn;
-    widen_species(w, r1, 0, tc, q);
-    dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-    jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
-    jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
-    widen_species(w + 9, r1, 1, tc, q);
-    dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-    widen_species(w + 18, r1, 2, tc, q);
-    dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-    const C rho = C(r1.o.X[0]) + dr;
-    const C rhoe = (dcp - dcn) + C(r1.o.X[1] - r1.o.X[2]);
-    const C Fx = C(cd->fcoef) * rhoe * C(Ex1 + cd->e_app) + C(cd->exf);
-    const C Fy = C(cd->fcoef) * rhoe * C(Ey1);
-    const C irho = C(1.0) / rho;
-    const C icfl = C(cd->inv_cfl), hdt = C(cd->half_dt);
-    mux = -((jx * icfl + Fx * hdt) * irho);
-    muy = -((jy * icfl + Fy * hdt) * irho);
-  }
-
   // fluid: pull gather (LBM.cu:869-905, x and y wrap); ions: the same, or
   // on a wall row the node's own values (LBM.cu:915-1023)
   float v[27];
@@ -570,9 +540,43 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   }
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
+  // y = 0: the mirror's inputs, node (x,1)'s pre-collision values and field,
+  // loaded together with the node's own
+  float w[27];
+  double Ex1 = 0.0, Ey1 = 0.0;
+  if (bottom) {
+    const long long r0 = nx, rm = 0, rp = 2LL * nx;
+#pragma unroll
+    for (int s = 0; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, w + 9 * s);
+    efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
+  }
   // every load is in flight before the first use: keep the scheduler from
   // interleaving the species' loads with the arithmetic
   __builtin_amdgcn_sched_barrier(0);
+
+  // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
+  // included (LBM.cu:402-443)
+  C mux = 0, muy = 0;
+  if (bottom) {
+    const Refs3 r1 = load_refs3(R, ny, 1);
+    C q[9], dr, jx, jy, dcp, dcn;
+    widen_species(w, r1, 0, tc, q);
+    dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
+    jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
+    widen_species(w + 9, r1, 1, tc, q);
+    dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    widen_species(w + 18, r1, 2, tc, q);
+    dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
+    const C rho = C(r1.o.X[0]) + dr;
+    const C rhoe = (dcp - dcn) + C(r1.o.X[1] - r1.o.X[2]);
+    const C Fx = C(cd->fcoef) * rhoe * C(Ex1 + cd->e_app) + C(cd->exf);
+    const C Fy = C(cd->fcoef) * rhoe * C(Ey1);
+    const C irho = C(1.0) / rho;
+    const C icfl = C(cd->inv_cfl), hdt = C(cd->half_dt);
+    mux = -((jx * icfl + Fx * hdt) * irho);
+    muy = -((jy * icfl + Fy * hdt) * irho);
+  }
 
   // moments, species by species (the expressions of moments() and of the
   // charge-density sums of k_lbm)
