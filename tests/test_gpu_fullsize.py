@@ -1,0 +1,82 @@
+"""The benchmark workloads at their full sizes against the FP64 oracle.
+
+A few coupled steps of the 4096 x 1024 channel (the bench workload) and of
+the 8192 x 2048 channel with zeta(x) walls, FP32 deviation storage with FP64
+arithmetic (the row LBM kernel, the one-tile x-FFTs, the tridiagonal at its
+real column height), compared field by field with the oracle started from the
+same seeded state; and fluid mass conservation over a few hundred steps of the
+full-size channel, a property that does not depend on the oracle.
+"""
+import numpy as np
+import pytest
+
+import ekpnp
+import oracle as O
+from helpers import field_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def wall_zeta(nx, seed, zeta0=-1.5, modes=4, amp=0.5):
+    """zeta(x) = zeta0 + sum_k a_k sin(2 pi k x / nx + phi_k), seeded numpy stream."""
+    rng = np.random.default_rng(seed)
+    a = rng.uniform(0.0, amp, modes)
+    phi = rng.uniform(0.0, 2 * np.pi, modes)
+    x = np.arange(nx)
+    return zeta0 + sum(a[k] * np.sin(2 * np.pi * (k + 1) * x / nx + phi[k]) for k in range(modes))
+
+
+def _run_pair(p, seed, nsteps, walls=None):
+    s = ekpnp.Solver([p], precision="fp32")
+    o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
+    if walls is not None:
+        s.set_wall_profile(0, **walls)
+        o.set_wall_profile(**walls)
+    s.initialization(ekpnp.COLD_NOISE, seed)
+    o.initialization(ekpnp.COLD_NOISE, seed, 0)
+    s.init_equilibrium()
+    o.init_equilibrium()
+    s.step(nsteps)
+    o.step(nsteps)
+    return s, o
+
+
+def test_bench_channel_4096x1024_against_the_oracle():
+    print("case 4096x1024 tau 0.6 zeta -2 kappa_h 32 E_x 5e-5 seed 1, 12 steps", flush=True)
+    p = ekpnp.lattice_params(4096, 1024, tau=0.6, zeta=-2.0, kappa_h=32.0, e_x=5e-5)
+    s, o = _run_pair(p, 1, 12)
+    errs = field_errors(s, 0, o)
+    print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    for f in ("rho", "ux", "cpos", "cneg", "psi"):
+        assert errs[f] < 1e-5, (f, errs[f])
+
+
+def test_wall_profile_channel_8192x2048_against_the_oracle():
+    print("case 8192x2048 tau 0.8 zeta(x) seed 4 kappa_h 64 E_x 1e-4, 4 steps", flush=True)
+    p = ekpnp.lattice_params(8192, 2048, tau=0.8, zeta=-1.5, kappa_h=64.0, e_x=1e-4)
+    z = wall_zeta(p.nx, 4)
+    walls = dict(psi_lo=z, psi_hi=z, cpos_lo=np.exp(-z), cpos_hi=np.exp(-z), cneg_lo=np.exp(z), cneg_hi=np.exp(z))
+    s, o = _run_pair(p, 4, 4, walls)
+    errs = field_errors(s, 0, o)
+    print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    for f in ("rho", "ux", "cpos", "cneg", "psi"):
+        assert errs[f] < 1e-5, (f, errs[f])
+
+
+def test_bench_channel_conserves_fluid_mass():
+    # full-way bounce-back walls and periodic x: the fluid mass is an exact
+    # invariant of the scheme; FP32 storage rounds each stored deviation, so
+    # it may drift only at that level
+    print("case 4096x1024 mass, 400 steps", flush=True)
+    p = ekpnp.lattice_params(4096, 1024, tau=0.6, zeta=-2.0, kappa_h=32.0, e_x=5e-5)
+    s = ekpnp.Solver([p], precision="fp32")
+    s.initialization(ekpnp.COLD_NOISE, 1)
+    s.init_equilibrium()
+    s.step(2)
+    m0 = s.reductions()[0, ekpnp.RED_NAMES.index("mass")]
+    s.step(400)
+    red = s.reductions()[0]
+    m1 = red[ekpnp.RED_NAMES.index("mass")]
+    print("mass", m0, m1, (m1 - m0) / m0, flush=True)
+    assert red[ekpnp.RED_NAMES.index("nonfinite")] == 0
+    assert abs(m1 - m0) <= 1e-7 * m0
