@@ -1,11 +1,17 @@
 // ekpnp_kernels.hip -- gfx950 kernels of the EK-PNP coupled time step.
 //
-// One step is four launches, each covering every channel of the context:
+// One step is an LBM stage and three Poisson launches, each launch covering
+// every channel of the context:
 //   k_lbm       fused pull collide-stream of f (fluid), h (c+), g (c-) with the
 //               wall rules folded in; writes the post-collision state of every
 //               node in place of the next lattice and the FP64 charge density
 //               (replaces gpu_collide_save, gpu_boundary, gpu_stream,
 //               gpu_bc_charge: LBM.cu:331-1024)
+//   k_lbm_rows  the same, same bits, with fewer registers for FP32 storage with
+//               FP64 arithmetic on channels whose rows are whole work-groups
+//               (one launch for their wall rows, one for their interior rows;
+//               k_lbm keeps the other channels and the first step after
+//               initialisation)
 //   k_fft_fwd   per x-row: right-hand side with the wall lifts, real-to-complex
 //               FFT in LDS (replaces odd_extension + forward cuFFT,
 //               poisson.cu:211-263, 190)
