@@ -621,19 +621,25 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
-  // the step's split: tiles of channels whose rows are whole work-groups go
-  // to the row kernel (FP32 storage, FP64 arithmetic); EKPNP_LBM_ROWS=0 in
-  // the environment keeps every tile on k_lbm
+  // the step's split (FP32 storage, FP64 arithmetic): channels with nx a
+  // multiple of 64 run on the row kernel, its interior launch taking the
+  // tiles that hold an interior row and its wall launch those that hold a
+  // wall row; the others run on k_lbm.  EKPNP_LBM_ROWS=0 in the environment
+  // keeps every tile on k_lbm.
   {
     const char* env = std::getenv("EKPNP_LBM_ROWS");
     const bool use_rows = !(env && env[0] == '0');
     std::vector<int2> wall, gen;
     for (const int2& t : c->lbm_tiles) {
       const ChanDesc& d = c->desc[t.x];
-      const int y = t.y / d.nx;
-      if (!use_rows || d.nx % LBM_WG != 0) gen.push_back(t);
-      else if (y == 0 || y == d.ny - 1) wall.push_back(t);
-      else c->lbm_split.push_back(t);
+      if (!use_rows || d.nx % 64 != 0) {
+        gen.push_back(t);
+        continue;
+      }
+      const int last = std::min(t.y + LBM_WG, d.nx * d.ny) - 1;  // last node of the tile
+      const int ya = t.y / d.nx, yb = last / d.nx;
+      if (ya == 0 || yb == d.ny - 1) wall.push_back(t);
+      if (!(ya == yb && (ya == 0 || ya == d.ny - 1))) c->lbm_split.push_back(t);
     }
     c->n_lbm_row = (int)c->lbm_split.size();
     c->n_lbm_wall = (int)wall.size();

@@ -71,9 +71,9 @@ struct StepBuffers {
   const ChanDesc* chans;
   const int2* lbm_tiles;        // (channel, first node) per LBM work-group
   int n_lbm_tiles;
-  // the same tiles split for the coupled step: channels whose rows are whole
-  // work-groups go to k_lbm_rows (interior rows, then wall rows), all the
-  // others to k_lbm
+  // the same tiles split for the coupled step: channels with nx a multiple
+  // of 64 go to k_lbm_rows (tiles with interior rows, tiles with wall rows),
+  // all the others to k_lbm
   const int2* lbm_row_tiles;
   int n_lbm_row_tiles;
   const int2* lbm_wall_tiles;

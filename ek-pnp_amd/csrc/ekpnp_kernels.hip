@@ -8,9 +8,9 @@
 //               (replaces gpu_collide_save, gpu_boundary, gpu_stream,
 //               gpu_bc_charge: LBM.cu:331-1024)
 //   k_lbm_rows  the same, same bits, with fewer registers for FP32 storage with
-//               FP64 arithmetic on channels whose rows are whole work-groups
-//               (one launch for their wall rows, one for their interior rows;
-//               k_lbm keeps the other channels and the first step after
+//               FP64 arithmetic on channels with nx a multiple of 64 (one
+//               launch for their wall rows, one for their interior rows; k_lbm
+//               keeps the other channels and the first step after
 //               initialisation)
 //   k_fft_fwd   per x-row: right-hand side with the wall lifts, real-to-complex
 //               FFT in LDS (replaces odd_extension + forward cuFFT,
@@ -448,11 +448,24 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
 // mirror, the ions' anti-bounce-back of the node's own post-collision values
 // and the fluid's full-way bounce-back.
 // ---------------------------------------------------------------------------
+// The row records a node reads (rows y-1, y, y+1: m, o, p), wave-uniform,
+// held in scalar registers
+struct RefsReg {
+  Refs3 r;
+  __device__ double mX(int s) const { return r.m.X[s]; }
+  __device__ double oX(int s) const { return r.o.X[s]; }
+  __device__ double pX(int s) const { return r.p.X[s]; }
+  __device__ double mJ(int s) const { return r.m.J[s]; }
+  __device__ double oJ(int s) const { return r.o.J[s]; }
+  __device__ double pJ(int s) const { return r.p.J[s]; }
+};
+
 // FP64 rest deviations of species sp from its gathered FP32 values (load_pre)
-__device__ __forceinline__ void widen_species(const float* v, const Refs3& rf, int sp, double tc, double* q) {
-  const double own = rf.o.X[sp];
-  const double dm = rf.m.X[sp] - own, dp = rf.p.X[sp] - own;
-  const double j0 = rf.o.J[sp] * tc, jm = rf.m.J[sp] * tc, jp = rf.p.J[sp] * tc;
+template <typename RF>
+__device__ __forceinline__ void widen_species(const float* v, const RF& rf, int sp, double tc, double* q) {
+  const double own = rf.oX(sp);
+  const double dm = rf.mX(sp) - own, dp = rf.pX(sp) - own;
+  const double j0 = rf.oJ(sp) * tc, jm = rf.mJ(sp) * tc, jp = rf.pJ(sp) * tc;
 #pragma unroll
   for (int d = 0; d < 9; ++d) q[d] = (double)v[d];
   q[1] += WS * j0;
@@ -483,40 +496,40 @@ __device__ __forceinline__ void abb_species(const float* p, double dcw, double j
 
 // the 9 values of species s at row y (s = 0: gathered; ions: gathered, or on
 // a wall row anti-bounced from the node's own values, dcw = c_w - X_s)
-__device__ __forceinline__ void species_q(const float* v, const Refs3& rf, int s, double tc, bool wall, double dcw,
+template <typename RF>
+__device__ __forceinline__ void species_q(const float* v, const RF& rf, int s, double tc, bool wall, double dcw,
                                           double* q) {
-  if (s > 0 && wall) abb_species(v, dcw, rf.o.J[s] * tc, q);
+  if (s > 0 && wall) abb_species(v, dcw, rf.oJ(s) * tc, q);
   else widen_species(v, rf, s, tc, q);
 }
 
 // 9 FP32 values of species block B gathered for node x of row y (pull,
-// periodic in x and y; rm, r0, rp are the row offsets of y-1, y, y+1)
-__device__ __forceinline__ void gather9(const float* __restrict__ B, long long plane, long long rm, long long r0,
-                                        long long rp, int x, int xm, int xp, float* v) {
-  v[0] = ld_pop(B + 0 * plane + r0 + x);
-  v[1] = ld_pop(B + 1 * plane + r0 + xm);
-  v[2] = ld_pop(B + 2 * plane + rm + x);
-  v[3] = ld_pop(B + 3 * plane + r0 + xp);
-  v[4] = ld_pop(B + 4 * plane + rp + x);
-  v[5] = ld_pop(B + 5 * plane + rm + xm);
-  v[6] = ld_pop(B + 6 * plane + rm + xp);
-  v[7] = ld_pop(B + 7 * plane + rp + xp);
-  v[8] = ld_pop(B + 8 * plane + rp + xm);
+// periodic in x and y; rm, r0, rp are the row offsets of y-1, y, y+1, which
+// fit 32 bits: a channel holds at most MAX_NX x (MAX_INTERIOR_ROWS + 2) nodes)
+__device__ __forceinline__ void gather9(const float* __restrict__ B, long long plane, int rm, int r0, int rp, int x,
+                                        int xm, int xp, float* v) {
+  v[0] = ld_pop(B + 0 * plane + (r0 + x));
+  v[1] = ld_pop(B + 1 * plane + (r0 + xm));
+  v[2] = ld_pop(B + 2 * plane + (rm + x));
+  v[3] = ld_pop(B + 3 * plane + (r0 + xp));
+  v[4] = ld_pop(B + 4 * plane + (rp + x));
+  v[5] = ld_pop(B + 5 * plane + (rm + xm));
+  v[6] = ld_pop(B + 6 * plane + (rm + xp));
+  v[7] = ld_pop(B + 7 * plane + (rp + xp));
+  v[8] = ld_pop(B + 8 * plane + (rp + xm));
 }
 
 // one node of a row tile (WALL: y = 0 or ny-1); a separate instantiation per
 // case keeps the interior code free of the wall rules' registers
-template <bool MACRO, bool WALL>
+template <bool MACRO, bool WALL, typename RF>
 __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDesc* __restrict__ cd, int y, int x,
-                                             const float* __restrict__ P, float* __restrict__ Q) {
+                                             const RF& rf, const float* __restrict__ P, float* __restrict__ Q) {
   using C = double;
   const int nx = cd->nx, ny = cd->ny;
   const long long node = (long long)y * nx + x;
   const long long plane = cd->plane;
   const double* __restrict__ ps = b.psi + cd->fld_off;
   const double* __restrict__ R = b.refs + cd->ref_off;
-  const Refs3 rf = load_refs3(R, ny, y);
-  const RowRefs& rr = rf.o;
   const double tc = cd->inv_cs2 * cd->inv_cfl;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
@@ -526,8 +539,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   // on a wall row the node's own values (LBM.cu:915-1023)
   float v[27];
   {
-    const long long r0 = (long long)y * nx;
-    const long long rm = (long long)(bottom ? ny - 1 : y - 1) * nx, rp = (long long)(top ? 0 : y + 1) * nx;
+    const int r0 = y * nx;
+    const int rm = (bottom ? ny - 1 : y - 1) * nx, rp = (top ? 0 : y + 1) * nx;
     gather9(P, plane, rm, r0, rp, x, xm, xp, v);
     if (wall) {
 #pragma unroll
@@ -541,8 +554,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   if (wall) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
-    dcwp = (bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws]) - rr.X[1];
-    dcwn = (bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws]) - rr.X[2];
+    dcwp = (bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws]) - rf.oX(1);
+    dcwn = (bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws]) - rf.oX(2);
   }
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
@@ -551,7 +564,7 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   float w[27];
   double Ex1 = 0.0, Ey1 = 0.0;
   if (bottom) {
-    const long long r0 = nx, rm = 0, rp = 2LL * nx;
+    const int r0 = nx, rm = 0, rp = 2 * nx;
 #pragma unroll
     for (int s = 0; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, w + 9 * s);
     efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
@@ -564,7 +577,7 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   // included (LBM.cu:402-443)
   C mux = 0, muy = 0;
   if (bottom) {
-    const Refs3 r1 = load_refs3(R, ny, 1);
+    const RefsReg r1{load_refs3(R, ny, 1)};
     C q[9], dr, jx, jy, dcp, dcn;
     widen_species(w, r1, 0, tc, q);
     dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
@@ -574,8 +587,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
     widen_species(w + 18, r1, 2, tc, q);
     dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-    const C rho = C(r1.o.X[0]) + dr;
-    const C rhoe = (dcp - dcn) + C(r1.o.X[1] - r1.o.X[2]);
+    const C rho = C(r1.oX(0)) + dr;
+    const C rhoe = (dcp - dcn) + C(r1.oX(1) - r1.oX(2));
     const C Fx = C(cd->fcoef) * rhoe * C(Ex1 + cd->e_app) + C(cd->exf);
     const C Fy = C(cd->fcoef) * rhoe * C(Ey1);
     const C irho = C(1.0) / rho;
@@ -603,8 +616,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
 #pragma unroll
     for (int k = 0; k < 9; ++k) sn += q[k];
   }
-  m.rho = C(rr.X[0]) + m.dr;
-  m.rhoe = (m.dcp - m.dcn) + C(rr.X[1] - rr.X[2]);
+  m.rho = C(rf.oX(0)) + m.dr;
+  m.rhoe = (m.dcp - m.dcn) + C(rf.oX(1) - rf.oX(2));
   m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
   m.Fy = C(cd->fcoef) * m.rhoe * C(Ey);
   {
@@ -614,14 +627,14 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     m.uy = (jy * icfl + m.Fy * hdt) * irho;
   }
   const C ux = bottom ? mux : m.ux, uy = bottom ? muy : m.uy;
-  b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.X[1] - rr.X[2]);
+  b.rho_e[cd->fld_off + node] = (sp - sn) + (rf.oX(1) - rf.oX(2));
   if (MACRO) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
     b.macro[M_RHO * st + o] = (double)m.rho;
     b.macro[M_UX * st + o] = (double)ux;
     b.macro[M_UY * st + o] = (double)uy;
-    b.macro[M_CP * st + o] = (double)(C(rr.X[1]) + m.dcp);
-    b.macro[M_CN * st + o] = (double)(C(rr.X[2]) + m.dcn);
+    b.macro[M_CP * st + o] = (double)(C(rf.oX(1)) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(rf.oX(2)) + m.dcn);
   }
 
   const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
@@ -632,23 +645,23 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     const C om_f = C(-0.5) * (ux * ux + uy * uy) * ics;
     C q[9];
     {
-      const C cp = C(rr.X[1]) + m.dcp;
+      const C cp = C(rf.oX(1)) + m.dcp;
       const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
       const C om_p = C(-0.5) * (vpx * vpx + vpy * vpy) * ics;
       species_q(v + 9, rf, 1, tc, wall, dcwp, q);
       collide_ion(q, m.dcp, cp, vpx, vpy, C(cd->om_pp), C(cd->om_mp), ics, ticf, om_p);
-      const C j = C(rr.J[1] * tc);
+      const C j = C(rf.oJ(1) * tc);
 #pragma unroll
       for (int d = 0; d < 9; ++d) st_pop(Q + (9 + d) * plane + node, (float)to_stored(q[d], d, j));
     }
     {
-      const C cn = C(rr.X[2]) + m.dcn;
+      const C cn = C(rf.oX(2)) + m.dcn;
       const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
       const C om_n = C(-0.5) * (vnx * vnx + vny * vny) * ics;
       species_q(v + 18, rf, 2, tc, wall, dcwn, q);
       collide_ion(q, m.dcn, cn, vnx, vny, C(cd->om_pn), C(cd->om_mn), ics, ticf,
                   (cd->flags & FLAG_G0_FLUID_U) ? om_f : om_n);
-      const C j = C(rr.J[2] * tc);
+      const C j = C(rf.oJ(2) * tc);
 #pragma unroll
       for (int d = 0; d < 9; ++d) st_pop(Q + (18 + d) * plane + node, (float)to_stored(q[d], d, j));
     }
@@ -674,14 +687,17 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
       collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
                     C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
     }
-    const C j = C(rr.J[0] * tc);
+    const C j = C(rf.oJ(0) * tc);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop(Q + d * plane + node, (float)to_stored(q[d], d, j));
   }
 }
 
-// WALL: the tiles are wall rows, else interior rows (two launches: one
-// kernel holding both paths would need the registers of both at once)
+// Channels with nx a multiple of 64: the 64 nodes of a wave lie in one row,
+// so the row index and its records are wave-uniform (scalar).  WALL: this
+// launch updates the waves on wall rows, the other one the waves on interior
+// rows (one kernel holding both paths would need the registers of both); a
+// tile holding both kinds of row is in both launches.
 template <bool MACRO, bool WALL>
 __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
                                                     const float* __restrict__ src, float* __restrict__ dst) {
@@ -689,13 +705,19 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
   const int2 tile = tiles[wg];
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
   if (!cd->active) return;  // converged channel, frozen by the residual driver
-  const int nx = cd->nx;
-  const int y = tile.y / nx;  // the work-group's row
-  const int x = tile.y - y * nx + (int)threadIdx.x;
+  const int nx = cd->nx, ny = cd->ny;
+  const int node0 = tile.y + ((int)threadIdx.x & ~63);  // the wave's first node
+  if (node0 >= nx * ny) return;                        // whole waves past the channel's last row
+  const int y = __builtin_amdgcn_readfirstlane(node0 / nx);
+  if ((y == 0 || y == ny - 1) != WALL) return;         // the other launch's wave
+  const int x = tile.y + (int)threadIdx.x - y * nx;
   const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
-  lbm_row_node<MACRO, WALL>(b, cd, y, x, P, Q);
+  const RefsReg rf{load_refs3(b.refs + cd->ref_off, ny, y)};
+  lbm_row_node<MACRO, WALL>(b, cd, y, x, rf, P, Q);
 }
+
+
 
 // ---------------------------------------------------------------------------
 // k_init_eq: equilibrium populations of the stored fields, unshifted
@@ -1880,7 +1902,8 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, co
     return hipGetLastError();
   }
   // the row kernel exists for FP32 storage with FP64 arithmetic
-  const bool rows = std::is_same<S, float>::value && std::is_same<C, double>::value && b.n_lbm_row_tiles > 0;
+  const bool rows = std::is_same<S, float>::value && std::is_same<C, double>::value &&
+                    (b.n_lbm_row_tiles > 0 || b.n_lbm_wall_tiles > 0);
   const int2* gen = rows ? b.lbm_gen_tiles : b.lbm_tiles;
   const int n_gen = rows ? b.n_lbm_gen_tiles : b.n_lbm_tiles;
   if (n_gen > 0) {
@@ -1896,9 +1919,11 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, co
       if (macro) hipLaunchKernelGGL((k_lbm_rows<true, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
       else hipLaunchKernelGGL((k_lbm_rows<false, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
     }
-    const dim3 grid(b.n_lbm_row_tiles);
-    if (macro) hipLaunchKernelGGL((k_lbm_rows<true, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
-    else hipLaunchKernelGGL((k_lbm_rows<false, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+    if (b.n_lbm_row_tiles > 0) {
+      const dim3 grid(b.n_lbm_row_tiles);
+      if (macro) hipLaunchKernelGGL((k_lbm_rows<true, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+      else hipLaunchKernelGGL((k_lbm_rows<false, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
+    }
   }
   return hipGetLastError();
 }

@@ -187,12 +187,13 @@ def test_fp32_arithmetic_variant_is_bounded():
     assert errs["ux"] < 1e-3
 
 
-@pytest.mark.parametrize("nx,ny", [(256, 40), (512, 24)])
+@pytest.mark.parametrize("nx,ny", [(256, 40), (512, 24), (64, 37), (128, 20), (192, 9), (320, 5)])
 def test_row_kernel_is_bit_identical_to_the_general_kernel(nx, ny, monkeypatch):
-    # interior rows of channels whose rows are whole work-groups run on
-    # k_lbm_rows (FP32 storage, FP64 arithmetic); EKPNP_LBM_ROWS=0 sends every
-    # tile to k_lbm.  Same operations in the same order: identical bits,
-    # also with the macroscopic output of the last step and across a rebase.
+    # channels with nx a multiple of 64 run on k_lbm_rows (FP32 storage, FP64
+    # arithmetic; a wave's nodes share a row, tiles may hold several rows or
+    # parts of two); EKPNP_LBM_ROWS=0 sends every tile to k_lbm.  Same
+    # operations in the same order: identical bits, also with the macroscopic
+    # output of the last step and across a rebase.
     p = ekpnp.lattice_params(nx, ny, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4)
     runs = []
     for env in ("1", "0"):
