@@ -30,7 +30,7 @@ const int CXH[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 // storage references: one record of 8 doubles per row, X (rho, c+, c-),
 // J (their x-fluxes), 2 unused
 constexpr int kRefStride = 8;
-constexpr int kMaxTriChunks = TRI_WG / 4;  // the plan keeps G = TRI_WG / C >= 4 modes per work-group
+constexpr int kMaxTriChunks = TRI_WG / 2;  // the plan keeps G = TRI_WG / C >= 2 modes per work-group
 
 // stored value + stored_ref(y, d) = population (FP32 storage; 0 for FP64)
 std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err);
@@ -90,7 +90,7 @@ struct ekpnp_ctx {
   double* d_partials = nullptr;
   double* d_red = nullptr;
   double* d_cprime = nullptr;    // chunk-entry Thomas coefficients (Poisson plan)
-  long long total_cprime = 0;    // its capacity: TRI_WG / 4 chunks x nk per channel
+  long long total_cprime = 0;    // its capacity: kMaxTriChunks x nk per channel
   size_t dev_bytes = 0;
   int max_lds = 65536;
   hipStream_t stream = nullptr;
@@ -647,10 +647,10 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     c->lbm_split.insert(c->lbm_split.end(), gen.begin(), gen.end());
   }
   // Poisson tile tables: capacity for the finest decomposition (one row per
-  // FFT tile, four modes per tridiagonal work-group)
+  // FFT tile, two modes per tridiagonal work-group)
   for (int i = 0; i < n_channels; ++i) {
     c->fft_tiles_cap += (size_t)(c->desc[i].ny - 2);
-    c->tri_tiles_cap += (size_t)((c->desc[i].nk + 3) / 4);
+    c->tri_tiles_cap += (size_t)((c->desc[i].nk + 1) / 2);
   }
 
   // wall planes and twiddles

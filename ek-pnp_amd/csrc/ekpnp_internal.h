@@ -27,7 +27,7 @@ constexpr int FFT_PAIR_SLOTS = (FFT_MAX_ELEMS / 2 + FFT_MAX_ROWS + FFT_WG_ROWS -
 constexpr int FFT_PF = 8;
 constexpr int FFT_PERSIST_ELEMS = FFT_PF * FFT_WG;  // 2048
 constexpr unsigned FFT_LDS_PERSIST = (FFT_PERSIST_ELEMS + FFT_PERSIST_ELEMS / 2) * 16;
-constexpr int TRI_WG = 512;       // lanes of a tridiagonal work-group (modes x chunks)
+constexpr int TRI_WG = 256;       // lanes of a tridiagonal work-group (modes x chunks)
 constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64)
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
 constexpr int TRI_LMAX = 16;              // rows per tridiagonal chunk (one lane)

@@ -1543,7 +1543,8 @@ template <int LMAX>
 __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
   __shared__ double sA[TRI_WG];
   __shared__ double2 sB[TRI_WG];
-  const int2 t = b.tri_tiles[blockIdx.x];
+  // consecutive tiles (the 8-mode lines of one spectrum block) share an XCD's L2
+  const int2 t = b.tri_tiles[xcd_remap(blockIdx.x, gridDim.x)];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
   if (!cd->active) return;
   const int G = cd->tri_G, L = cd->tri_L;
