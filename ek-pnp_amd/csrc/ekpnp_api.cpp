@@ -74,6 +74,9 @@ struct ekpnp_ctx {
   int2* d_lbm_split = nullptr;   // lbm_split on the device
   std::vector<int2> lbm_split;   // LBM tiles for the step: row-kernel interior, row-kernel wall, others
   int n_lbm_row = 0, n_lbm_wall = 0;
+  // the coupled step updates the row kernel's wall tiles inside the
+  // tridiagonal launch (set with the Poisson plan)
+  bool walls_in_tri = false;
   int4* d_fft_tiles = nullptr;
   int2* d_tri_tiles = nullptr;
   int* d_first_tile = nullptr;
@@ -342,6 +345,16 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   }
   c->sb.fft_persist = persist_ok ? 1 : 0;
   c->sb.fft_grid = std::max(1, std::min(n_fft_pow2, 2 * c->n_cu));
+  // Wall tiles on the tridiagonal launch when they fit the slots its tiles
+  // leave free (256-lane work-groups, three per CU at its register count):
+  // the launch then keeps its number of rounds.  EKPNP_WALLS_IN_TRI=0 in the
+  // environment keeps them on a launch of their own.
+  {
+    const char* env = std::getenv("EKPNP_WALLS_IN_TRI");
+    const long long slots = 3LL * c->n_cu;
+    const long long nt = (long long)c->tri_tiles.size(), nw = c->n_lbm_wall;
+    c->walls_in_tri = !(env && env[0] == '0') && nw > 0 && (nt + nw + slots - 1) / slots == (nt + slots - 1) / slots;
+  }
   c->fft_elems = fft_elems;
   c->tri_rows = tri_rows;
   c->fft_persist = c->sb.fft_persist;
@@ -367,20 +380,43 @@ hipError_t enqueue_poisson(ekpnp_ctx* c, int only_channel) {
   return launch_poisson_inv(b, c->stream);
 }
 
-hipError_t enqueue_lbm(ekpnp_ctx* c, bool macro) {
+hipError_t enqueue_lbm(ekpnp_ctx* c, bool macro, bool walls = true) {
   hipError_t e = launch_lbm(c->opt.precision, c->opt.compute_fp64 != 0, c->state_q, macro, c->sb,
                             c->lat[c->cur],
-                            c->lat[c->cur ^ 1], c->stream);
+                            c->lat[c->cur ^ 1], c->stream, walls);
   c->cur ^= 1;
   c->state_q = false;
   return e;
 }
 
+// whether the next step's wall tiles go on its tridiagonal launch: row-kernel
+// steps only (FP32 storage, FP64 arithmetic, populations not in the
+// pre-collision form of a restart)
+bool defer_walls(const ekpnp_ctx* c) {
+  return c->walls_in_tri && !c->state_q && c->opt.precision == 0 && c->opt.compute_fp64 != 0;
+}
+
+// one coupled step: collide-stream, then the Poisson stages; ev (optional)
+// receives an event after each of the four launches
+hipError_t enqueue_step(ekpnp_ctx* c, bool macro, hipEvent_t* ev = nullptr) {
+  const bool defer = defer_walls(c);
+  const void* src = c->lat[c->cur];
+  void* dst = c->lat[c->cur ^ 1];
+  hipError_t e = enqueue_lbm(c, macro, !defer);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[0], c->stream);
+  if (e == hipSuccess) e = launch_poisson_fwd(c->sb, c->stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[1], c->stream);
+  if (e == hipSuccess)
+    e = defer ? launch_poisson_tri_walls(macro, c->sb, src, dst, c->stream) : launch_poisson_tri(c->sb, c->stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[2], c->stream);
+  if (e == hipSuccess) e = launch_poisson_inv(c->sb, c->stream);
+  if (e == hipSuccess && ev) e = hipEventRecord(ev[3], c->stream);
+  return e;
+}
+
 hipError_t enqueue_steps(ekpnp_ctx* c, int n, bool macro_last) {
   for (int i = 0; i < n; ++i) {
-    hipError_t e = enqueue_lbm(c, macro_last && i == n - 1);
-    if (e != hipSuccess) return e;
-    e = enqueue_poisson(c, -1);
+    hipError_t e = enqueue_step(c, macro_last && i == n - 1);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -1272,6 +1308,15 @@ int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_
   return EKPNP_OK;
 }
 
+int ekpnp_get_poisson_plan(const ekpnp_ctx* ctx, int* out) {
+  if (!ctx || !out) return EKPNP_ERR_INVALID;
+  out[0] = ctx->fft_elems;
+  out[1] = ctx->tri_rows;
+  out[2] = ctx->fft_persist;
+  out[3] = ctx->walls_in_tri ? 1 : 0;
+  return EKPNP_OK;
+}
+
 int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {
   int rc = check_channel(ctx, channel);
   if (rc) return rc;
@@ -1424,15 +1469,10 @@ int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out) {
   double acc[4] = {0, 0, 0, 0};
   int rc = EKPNP_OK;
   for (int it = 0; it < nsteps && rc == EKPNP_OK; ++it) {
+    // the step as the graphs enqueue it (wall tiles on the tridiagonal
+    // launch when the plan puts them there)
     hipError_t e = hipEventRecord(ev[0], ctx->stream);
-    if (e == hipSuccess) e = enqueue_lbm(ctx, false);
-    if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
-    if (e == hipSuccess) e = launch_poisson_fwd(ctx->sb, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[2], ctx->stream);
-    if (e == hipSuccess) e = launch_poisson_tri(ctx->sb, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[3], ctx->stream);
-    if (e == hipSuccess) e = launch_poisson_inv(ctx->sb, ctx->stream);
-    if (e == hipSuccess) e = hipEventRecord(ev[4], ctx->stream);
+    if (e == hipSuccess) e = enqueue_step(ctx, false, ev + 1);
     if (e == hipSuccess) e = hipEventSynchronize(ev[4]);
     if (e != hipSuccess) {
       rc = fail(ctx, EKPNP_ERR_HIP, std::string("time_stages: ") + hipGetErrorString(e));

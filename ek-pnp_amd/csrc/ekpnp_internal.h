@@ -113,10 +113,15 @@ struct StepBuffers {
 // ---- launchers (ekpnp_kernels.hip) ----------------------------------------
 // precision: 0 FP32 deviations, 1 FP64; compute64: FP64 collision arithmetic
 // for FP32 storage
+// walls = false: the row kernel's wall tiles are left out (for
+// launch_poisson_tri_walls to update them)
 hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, const StepBuffers& b,
-                      const void* src, void* dst, hipStream_t s);
+                      const void* src, void* dst, hipStream_t s, bool walls = true);
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s);
 hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s);
+// the tridiagonal plus the row kernel's wall tiles of the collide-stream that
+// read src and write dst (FP32 storage, FP64 arithmetic)
+hipError_t launch_poisson_tri_walls(bool macro, const StepBuffers& b, const void* src, void* dst, hipStream_t s);
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
 // chunk-entry Thomas coefficients of channel chan for its current tri_C, tri_L
 hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s);

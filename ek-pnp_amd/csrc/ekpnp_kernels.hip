@@ -18,7 +18,8 @@
 //   k_tridiag   per wavenumber: the wall-normal tridiagonal solve as a chunked
 //               Thomas sweep joined by affine scans in LDS (replaces
 //               gpu_derivative and the y half of the 2-D cuFFT,
-//               poisson.cu:265-283)
+//               poisson.cu:265-283); in a coupled step k_tridiag_walls,
+//               which also runs that step's k_lbm_rows wall rows
 //   k_fft_inv   per x-row: complex-to-real FFT, writes psi (replaces inverse
 //               cuFFT + odd_extract, poisson.cu:196, 285-306)
 // The electric field is never stored: k_lbm differentiates psi on the fly
@@ -699,10 +700,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
 // rows (one kernel holding both paths would need the registers of both); a
 // tile holding both kinds of row is in both launches.
 template <bool MACRO, bool WALL>
-__global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
-                                                    const float* __restrict__ src, float* __restrict__ dst) {
-  const int wg = xcd_remap(blockIdx.x, gridDim.x);
-  const int2 tile = tiles[wg];
+__device__ __forceinline__ void lbm_rows_tile(const StepBuffers& b, int2 tile, const float* __restrict__ src,
+                                              float* __restrict__ dst) {
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
   if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
@@ -715,6 +714,12 @@ __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* 
   float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
   const RefsReg rf{load_refs3(b.refs + cd->ref_off, ny, y)};
   lbm_row_node<MACRO, WALL>(b, cd, y, x, rf, P, Q);
+}
+
+template <bool MACRO, bool WALL>
+__global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
+                                                    const float* __restrict__ src, float* __restrict__ dst) {
+  lbm_rows_tile<MACRO, WALL>(b, tiles[xcd_remap(blockIdx.x, gridDim.x)], src, dst);
 }
 
 
@@ -1540,11 +1545,7 @@ __device__ __forceinline__ double2 chunk_scan(double A, double2 B, int G, int wv
 }
 
 template <int LMAX>
-__global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
-  __shared__ double sA[TRI_WG];
-  __shared__ double2 sB[TRI_WG];
-  // consecutive tiles (the 8-mode lines of one spectrum block) share an XCD's L2
-  const int2 t = b.tri_tiles[xcd_remap(blockIdx.x, gridDim.x)];
+__device__ __forceinline__ void tridiag_tile(const StepBuffers& b, int2 t, double* sA, double2* sB) {
   const ChanDesc* __restrict__ cd = b.chans + t.x;
   if (!cd->active) return;
   const int G = cd->tri_G, L = cd->tri_L;
@@ -1628,6 +1629,35 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
     for (int ii = 0; ii < LMAX; ++ii)
       if (ii < len) col[(long long)(i0 + ii) * ks] = make_double2(yr[ii], yi[ii]);
   }
+}
+
+template <int LMAX>
+__global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
+  __shared__ double sA[TRI_WG];
+  __shared__ double2 sB[TRI_WG];
+  // consecutive tiles (the 8-mode lines of one spectrum block) share an XCD's L2
+  tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(blockIdx.x, gridDim.x)], sA, sB);
+}
+
+// The tridiagonal launch of a coupled step with the wall rows of the step's
+// collide-stream appended: work-groups past the tridiagonal tiles update the
+// row-kernel wall tiles (as k_lbm_rows<MACRO, true>).  Valid because the
+// wall-row update reads only the step's source populations, psi and the
+// storage references, none of which the forward x-FFT or the tridiagonal
+// writes, and writes rows 0 and ny-1, which neither of them reads; the wall
+// path needs fewer registers than the tridiagonal, so the launch keeps its
+// occupancy, and its work-groups fill slots the tridiagonal leaves free.
+static_assert(TRI_WG == LBM_WG, "wall tiles are LBM work-group sized");
+template <int LMAX, bool MACRO>
+__global__ __launch_bounds__(TRI_WG) void k_tridiag_walls(StepBuffers b, const float* __restrict__ src,
+                                                          float* __restrict__ dst) {
+  __shared__ double sA[TRI_WG];
+  __shared__ double2 sB[TRI_WG];
+  const int nt = b.n_tri_tiles;
+  if ((int)blockIdx.x < nt)
+    tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(blockIdx.x, nt)], sA, sB);
+  else
+    lbm_rows_tile<MACRO, true>(b, b.lbm_wall_tiles[blockIdx.x - nt], src, dst);
 }
 
 // table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
@@ -1891,7 +1921,7 @@ __global__ __launch_bounds__(256) void k_psi_wall_rows(StepBuffers b, int chan) 
 // launchers
 // ---------------------------------------------------------------------------
 template <typename S, typename C>
-static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, const void* src,
+static hipError_t lbm_dispatch(bool from_q, bool macro, bool walls, const StepBuffers& b, const void* src,
                                void* dst, hipStream_t s) {
   const S* sp = static_cast<const S*>(src);
   S* dp = static_cast<S*>(dst);
@@ -1915,7 +1945,7 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, co
   if (rows) {
     const float* fs = reinterpret_cast<const float*>(src);
     float* fd = reinterpret_cast<float*>(dst);
-    if (b.n_lbm_wall_tiles > 0) {
+    if (walls && b.n_lbm_wall_tiles > 0) {
       const dim3 grid(b.n_lbm_wall_tiles);
       if (macro) hipLaunchKernelGGL((k_lbm_rows<true, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
       else hipLaunchKernelGGL((k_lbm_rows<false, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
@@ -1930,12 +1960,12 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, const StepBuffers& b, co
 }
 
 hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, const StepBuffers& b,
-                      const void* src, void* dst, hipStream_t s) {
+                      const void* src, void* dst, hipStream_t s, bool walls) {
   if (precision == 0) {
-    if (compute64) return lbm_dispatch<float, double>(from_q, macro, b, src, dst, s);
-    return lbm_dispatch<float, float>(from_q, macro, b, src, dst, s);
+    if (compute64) return lbm_dispatch<float, double>(from_q, macro, walls, b, src, dst, s);
+    return lbm_dispatch<float, float>(from_q, macro, walls, b, src, dst, s);
   }
-  return lbm_dispatch<double, double>(from_q, macro, b, src, dst, s);
+  return lbm_dispatch<double, double>(from_q, macro, walls, b, src, dst, s);
 }
 
 // The FFT tile table holds the power-of-two tiles first (n_fft_pow2 of them),
@@ -1980,6 +2010,15 @@ hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk
 hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s) {
   // the plan keeps every chunk at <= TRI_LMAX rows
   hipLaunchKernelGGL((k_tridiag<TRI_LMAX>), dim3(b.n_tri_tiles), dim3(TRI_WG), 0, s, b);
+  return hipGetLastError();
+}
+
+hipError_t launch_poisson_tri_walls(bool macro, const StepBuffers& b, const void* src, void* dst, hipStream_t s) {
+  const float* fs = static_cast<const float*>(src);
+  float* fd = static_cast<float*>(dst);
+  const dim3 grid(b.n_tri_tiles + b.n_lbm_wall_tiles);
+  if (macro) hipLaunchKernelGGL((k_tridiag_walls<TRI_LMAX, true>), grid, dim3(TRI_WG), 0, s, b, fs, fd);
+  else hipLaunchKernelGGL((k_tridiag_walls<TRI_LMAX, false>), grid, dim3(TRI_WG), 0, s, b, fs, fd);
   return hipGetLastError();
 }
 

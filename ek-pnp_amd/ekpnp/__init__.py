@@ -117,6 +117,7 @@ SIGNATURES = [
     ("ekpnp_get_params", ctypes.c_int, [_VP, ctypes.c_int, _P(ChannelParams)]),
     ("ekpnp_set_poisson_plan", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("ekpnp_autotune", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
+    ("ekpnp_get_poisson_plan", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
     ("ekpnp_num_channels", ctypes.c_int, [_VP]),
     ("ekpnp_steps_done", ctypes.c_longlong, [_VP]),
     ("ekpnp_device_bytes", ctypes.c_size_t, [_VP]),
@@ -391,6 +392,12 @@ class Solver:
         self._check(lib().ekpnp_autotune(self._ctx, int(reps), _dptr(out)))
         return {"fft_elems": int(out[0]), "tri_rows": int(out[1]), "poisson_ms": float(out[2]),
                 "persistent": bool(out[3])}
+
+    def poisson_plan(self) -> dict:
+        """The plan in force (ekpnp_get_poisson_plan)."""
+        out = (ctypes.c_int * 4)()
+        self._check(lib().ekpnp_get_poisson_plan(self._ctx, out))
+        return {"fft_elems": out[0], "tri_rows": out[1], "persistent": bool(out[2]), "walls_in_tri": bool(out[3])}
 
     def get_params(self, ch: int) -> ChannelParams:
         out = ChannelParams()

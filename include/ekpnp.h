@@ -289,6 +289,15 @@ int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_
  * per solve, persistent}.  No counterpart in the reference (cuFFT plans are
  * fixed by main.cu:113). */
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
+/* The plan in force: out[0] fft_elems, out[1] tri_rows, out[2] persistent FFT
+ * kernels in use, out[3] whether a coupled step updates the collide-stream's
+ * wall rows (the rows y = 0 and ny - 1 of channels with nx a multiple of 64)
+ * inside its tridiagonal launch instead of a launch of their own.  That
+ * placement is chosen with the plan, when the wall work-groups fit beside
+ * the tridiagonal's without another round; EKPNP_WALLS_IN_TRI=0 in the
+ * environment at plan time turns it off.  Results do not depend on it.  No
+ * counterpart in the reference. */
+int ekpnp_get_poisson_plan(const ekpnp_ctx* ctx, int* out);
 
 /* ---- introspection / timing -------------------------------------------------- */
 /* The parameters channel was created with (wall profiles not included). */
@@ -299,7 +308,10 @@ long long ekpnp_steps_done(const ekpnp_ctx* ctx);
 size_t ekpnp_device_bytes(const ekpnp_ctx* ctx);
 /* Times nsteps eager steps with HIP events on the context's stream, per
  * stage: ms_out[0] LBM, [1] forward x-FFT, [2] wall-normal tridiagonal,
- * [3] inverse x-FFT, [4] whole step (graph replay of nsteps, per step). */
+ * [3] inverse x-FFT, [4] whole step (graph replay of nsteps, per step).  The
+ * launches are those of the coupled step: when the plan puts the wall rows
+ * on the tridiagonal launch (ekpnp_get_poisson_plan), [0] covers the other
+ * rows and [2] includes the wall rows. */
 int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out);
 
 #ifdef __cplusplus

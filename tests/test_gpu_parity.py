@@ -210,6 +210,42 @@ def test_row_kernel_is_bit_identical_to_the_general_kernel(nx, ny, monkeypatch):
         np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=k)
 
 
+def test_wall_rows_on_the_tridiagonal_launch_are_bit_identical(monkeypatch):
+    # The coupled step may update the row kernel's wall rows inside the
+    # tridiagonal launch (k_tridiag_walls) instead of a launch of their own;
+    # EKPNP_WALLS_IN_TRI=0 at plan time keeps the separate launch.  A batch of
+    # row-kernel channels (one tile spanning several rows, one wide) and a
+    # k_lbm channel, stepped through the graphs (macroscopic output on the
+    # last step), the per-operator calls and a time_stages pass: identical
+    # bits both ways.
+    ps = [ekpnp.lattice_params(64, 30, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4),
+          ekpnp.lattice_params(512, 20, tau=0.8, zeta=-1.0, kappa_h=4.0, e_x=1e-4),
+          ekpnp.lattice_params(48, 26, tau=0.9, zeta=-2.0, kappa_h=5.0, e_x=3e-4)]
+    runs = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("EKPNP_WALLS_IN_TRI", env)
+        s = ekpnp.Solver(ps, precision="fp32")
+        assert s.poisson_plan()["walls_in_tri"] == (env == "1")
+        s.initialization(ekpnp.COLD_NOISE, 9)
+        s.init_equilibrium()
+        s.step(97)
+        s.stream_collide_save()
+        s.fast_poisson()
+        s.time_stages(3)
+        s.step(64)
+        out = {}
+        for ch in range(len(ps)):
+            for sp in ("f", "h", "g"):
+                out[(ch, sp)] = s.get_populations(ch, sp)
+            for f in ("rho", "ux", "uy", "cpos", "cneg", "psi"):
+                out[(ch, f)] = s.get_field(ch, f)
+        out["red"] = s.reductions()
+        runs.append(out)
+        s.close()
+    for k in runs[0]:
+        np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=str(k))
+
+
 def test_fp32_row_kernel_channel_against_the_oracle():
     # a 256-wide FP32 channel (row kernel) and a 96-wide one (k_lbm) in one
     # batch, each against its FP64 oracle after 2000 steps
