@@ -822,13 +822,21 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 // Twiddles: LDS holds e^{-2 pi i j/nx} for j < nx/4 (one quadrant); the rest
 // of [0, nx/2] follows from e^{-2 pi i (j + nx/4)/nx} = -i e^{-2 pi i j/nx}.
 // ---------------------------------------------------------------------------
+// Both helpers are branch-free: one table load at a clamped index, then
+// selects and sign-bit flips (exact negations), so lanes of different
+// quadrants do not split the wave.
+__device__ __forceinline__ double neg_if(double v, bool neg) {
+  return __longlong_as_double(__double_as_longlong(v) ^ ((long long)neg << 63));
+}
+
 __device__ __forceinline__ double2 tw_half(const double2* __restrict__ q, int j, int quarter) {
-  if (j < quarter) return q[j];
-  if (j < 2 * quarter) {
-    const double2 w = q[j - quarter];
-    return make_double2(w.y, -w.x);
-  }
-  return make_double2(-1.0, 0.0);  // j == nx/2
+  // j < nx/4: q[j]; j < nx/2: -i q[j - nx/4]; j == nx/2: -1
+  const bool up = j >= quarter;
+  const int jj = up ? j - quarter : j;
+  const double2 w = q[min(jj, quarter - 1)];
+  const bool last = jj == quarter;
+  const double re = up ? w.y : w.x, im = up ? neg_if(w.x, true) : w.y;
+  return make_double2(last ? -1.0 : re, last ? 0.0 : im);
 }
 
 // ---------------------------------------------------------------------------
@@ -917,15 +925,16 @@ __device__ __forceinline__ void dft_reg(double2* v) {
 // e^{SIGN 2 pi i t/nx} for 0 <= t < nx from the quadrant table
 template <int SIGN>
 __device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t, int lgnx) {
+  // t in [0, nx): quadrant 0..3 multiplies the table value by (-i)^quad:
+  // (x, y), (y, -x), (-x, -y), (-y, x); SIGN > 0 conjugates
   const int sh = lgnx - 2;
   const int quad = t >> sh;
   const double2 w = q[t & ((1 << sh) - 1)];
-  double2 r;
-  if (quad == 0) r = w;
-  else if (quad == 1) r = make_double2(w.y, -w.x);
-  else if (quad == 2) r = make_double2(-w.x, -w.y);
-  else r = make_double2(-w.y, w.x);
-  return SIGN > 0 ? cconj(r) : r;
+  const bool swap = quad & 1;
+  const double a = swap ? w.y : w.x, b = swap ? w.x : w.y;
+  const bool nre = (quad & 2) != 0;
+  const bool nim = (((quad ^ (quad >> 1)) & 1) != 0) != (SIGN > 0);
+  return make_double2(neg_if(a, nre), neg_if(b, nim));
 }
 
 // LDS position of row element i: bits 0-3 XOR bits 4-7.  The Stockham
