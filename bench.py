@@ -108,8 +108,20 @@ def measured_traffic(config, precision):
         return json.load(f).get("bytes_per_launch")
 
 
-def cpu_baseline(params, seconds_target=15.0):
-    """The FP64 oracle on the host cores, bounded sample of the same workload."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(params, seconds_target=15.0, single_seconds=6.0):
+    """The FP64 oracle on the host cores, bounded sample of the same workload;
+    then the same on one thread, a shorter sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
 
@@ -117,17 +129,27 @@ def cpu_baseline(params, seconds_target=15.0):
     o = O.Oracle(p)
     o.initialization(0, 1, 0)
     o.init_equilibrium()
+    threads = O.max_threads()
     t1 = o.time_steps(1)
     n = max(1, min(200, int(seconds_target / max(t1, 1e-6))))
     t = o.time_steps(n)
     nodes = p.nx * p.ny
+    O.set_threads(1)
+    t1s = o.time_steps(1)
+    n1 = max(1, min(50, int(single_seconds / max(t1s, 1e-6))))
+    ts = o.time_steps(n1)
+    O.set_threads(threads)
     return {
         "value": round(n * nodes / t / 1e6, 3),
         "unit": "MLUPS",
-        "cores": O.max_threads(),
+        "cores": threads,
         "kind": "port",
         "sample": f"FP64 oracle, {p.nx}x{p.ny} channel, {n} coupled steps after 1 warm-up step "
                   f"({t:.1f} s), OpenMP over rows",
+        "cpu": cpu_model(),
+        "host_cpus": os.cpu_count(),
+        "single_thread": {"value": round(n1 * nodes / ts / 1e6, 3), "unit": "MLUPS",
+                          "sample": f"{n1} coupled steps after 1 warm-up step ({ts:.1f} s), 1 thread"},
     }
 
 

@@ -335,7 +335,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   c->sb.tri_lmax = tri_lmax;
   // the persistent kernels take tiles of <= FFT_PERSIST_ELEMS values and keep
   // twiddles in LDS
-  bool persist_ok = fft_persist && fft_elems <= FFT_PERSIST_ELEMS && FFT_LDS_PERSIST <= (unsigned)c->max_lds;
+  bool persist_ok = fft_persist == 1 && fft_elems <= FFT_PERSIST_ELEMS && FFT_LDS_PERSIST <= (unsigned)c->max_lds;
   for (const auto& d : c->desc) {
     if (!d.pow2) continue;
     const int N = d.nx / 2;
@@ -345,6 +345,29 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   }
   c->sb.fft_persist = persist_ok ? 1 : 0;
   c->sb.fft_grid = std::max(1, std::min(n_fft_pow2, 2 * c->n_cu));
+  // row-pair kernels: every power-of-two channel of one width nx = 2 N with
+  // N = S * FFT_WG_ROWS, S in {2, 4}, two rows per tile (fft_elems = 2 N),
+  // rows and twiddles in LDS
+  c->sb.fft_pair_s = 0;
+  if (fft_persist == 2 && n_fft_pow2 > 0 && c->sb.fft_tw_lds) {
+    int n_common = -1;
+    bool ok = true;
+    for (const auto& d : c->desc) {
+      if (!d.pow2) continue;
+      const int N = d.nx / 2;
+      if (n_common < 0) n_common = N;
+      ok = ok && N == n_common;
+    }
+    const int S = n_common / FFT_WG_ROWS;
+    ok = ok && n_common % FFT_WG_ROWS == 0 && (S == 2 || S == 4) && fft_elems == 2 * n_common;
+    // two row regions and the quadrant table, also when no tile holds two rows
+    const unsigned lds = (unsigned)((2 * n_common + n_common / 2) * sizeof(double2));
+    ok = ok && lds <= (unsigned)c->max_lds;
+    if (ok) {
+      c->sb.fft_pair_s = S;
+      c->sb.fft_lds_pow2 = std::max(c->sb.fft_lds_pow2, lds);
+    }
+  }
   // Wall tiles on the tridiagonal launch when they fit the slots its tiles
   // leave free (256-lane work-groups, three per CU at its register count):
   // the launch then keeps its number of rounds.  EKPNP_WALLS_IN_TRI=0 in the
@@ -357,7 +380,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   }
   c->fft_elems = fft_elems;
   c->tri_rows = tri_rows;
-  c->fft_persist = c->sb.fft_persist;
+  c->fft_persist = c->sb.fft_pair_s > 0 ? 2 : c->sb.fft_persist;
   drop_graphs(c);
   return hipSuccess;
 }
@@ -1262,11 +1285,14 @@ int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
   const int rows[2] = {8, 16};
   int best_e = ctx->fft_elems, best_r = ctx->tri_rows, best_p = ctx->fft_persist;
   double best = 1e30;
-  for (int ic = 0; ic < 6; ++ic) {
-    const int ie = ic % 3, ip = ic / 3;  // classic kernels for every size, persistent up to 2048
+  for (int ic = 0; ic < 9; ++ic) {
+    // one-tile kernels for every size, persistent up to 2048, row pairs
+    // where the channels allow them
+    const int ie = ic % 3, ip = ic / 3;
     if (ip == 1 && elems[ie] > FFT_PERSIST_ELEMS) continue;
     for (int ir = 0; ir < 2; ++ir) {
       hipError_t e = build_poisson_plan(ctx, elems[ie], rows[ir], ip);
+      if (e == hipSuccess && ip == 2 && ctx->fft_persist != 2) break;  // not eligible
       for (int w = 0; w < 2 && e == hipSuccess; ++w) e = enqueue_poisson(ctx, -1);
       if (e == hipSuccess) e = hipEventRecord(ev[0], ctx->stream);
       for (int k = 0; k < reps && e == hipSuccess; ++k) e = enqueue_poisson(ctx, -1);
@@ -1300,11 +1326,13 @@ int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
 
 int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent) {
   if (!ctx) return EKPNP_ERR_INVALID;
-  if (fft_elems < 1 || fft_elems > FFT_MAX_ELEMS || tri_rows < 1 || tri_rows > 16)
-    return fail(ctx, EKPNP_ERR_INVALID, "fft_elems must be in [1, 4096] and tri_rows in [1, 16]");
+  if (fft_elems < 1 || fft_elems > FFT_MAX_ELEMS || tri_rows < 1 || tri_rows > 16 || fft_persistent < 0 ||
+      fft_persistent > 2)
+    return fail(ctx, EKPNP_ERR_INVALID,
+                "fft_elems must be in [1, 4096], tri_rows in [1, 16] and fft_persistent in [0, 2]");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-  HIP_OR_FAIL(ctx, build_poisson_plan(ctx, fft_elems, tri_rows, fft_persistent ? 1 : 0));
+  HIP_OR_FAIL(ctx, build_poisson_plan(ctx, fft_elems, tri_rows, fft_persistent));
   return EKPNP_OK;
 }
 

@@ -84,6 +84,7 @@ struct StepBuffers {
   int n_fft_tiles;
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
   int fft_persist;              // power-of-two tiles by the persistent kernels
+  int fft_pair_s;               // > 0: by the row-pair kernels, N = fft_pair_s * FFT_WG_ROWS for every one
   int fft_grid;                 // work-groups of the persistent kernels
   unsigned fft_lds_pow2;        // dynamic LDS bytes of the Stockham kernels
   unsigned fft_lds_dft;         // dynamic LDS bytes of the direct-DFT kernels

@@ -1481,6 +1481,186 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv(StepBuffers b, const in
 }
 
 // ---------------------------------------------------------------------------
+// Row-pair x-FFTs.  A work-group takes the rows A = y0 and B = y0 + 1 of a
+// tile (a channel's last tile may hold A only) and transforms them one after
+// the other.  Every load of both rows is issued before the first is used,
+// row B's after row A's, so row B's data arrives while row A is transformed,
+// and row A's stores drain while row B is transformed.  Loads and stores are
+// straight-line and unconditional (indices past the end are clamped: a
+// duplicate store writes the same value to the same place), so the wait the
+// compiler puts before each use counts exactly the memory operations issued
+// after the one needed; in particular row B's waits do not include row A's
+// stores.  S = N / FFT_WG_ROWS values per lane and row; twiddles pass
+// through registers into LDS with row A.
+// ---------------------------------------------------------------------------
+constexpr int clog2(int v) { return v <= 1 ? 0 : 1 + clog2(v >> 1); }
+
+template <int S>
+__global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd2(StepBuffers b, const int4* __restrict__ tiles,
+                                                          const ChanDesc* __restrict__ chans) {
+  extern __shared__ double2 lds[];
+  constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, quarter = N / 2;
+  constexpr int TQ = quarter / WG;  // twiddle values per lane
+  static_assert(S >= 2 && quarter % WG == 0, "row-pair FFT sizes");
+  const int4 t = tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
+  if (!cd->active) return;
+  const int nx = 2 * N, ny = cd->ny, y0 = t.y;
+  const bool two = t.z == 2;
+  const double* __restrict__ re = b.rho_e + cd->fld_off;
+  const double* __restrict__ Wl = b.walls + cd->wall_off;
+  const long long ws = b.walls_stride;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  double2* __restrict__ spec = b.spec + cd->spec_off;
+  const double pc = cd->pc_dy2;
+  const int tid = threadIdx.x;
+  double2* rowA = lds;
+  double2* rowB = lds + N;
+  double2* twl = lds + 2 * N;
+  // wall lifts (odd_extension, poisson.cu:229-241): row 1 takes psi_lo, row
+  // ny-2 psi_hi; loaded first, only by the tiles that hold those rows, and
+  // read only under the same conditions (left unset otherwise, so that no
+  // merge of the loaded values makes the other tiles wait for them)
+  const bool wlo = y0 == 1, whi = y0 + t.z - 1 == ny - 2;
+  double2 wl[S], wh[S];
+  if (wlo) {
+    const double2* w = reinterpret_cast<const double2*>(Wl + W_PSI_LO * ws);
+#pragma unroll
+    for (int c = 0; c < S; ++c) wl[c] = w[tid + c * WG];
+  }
+  if (whi) {
+    const double2* w = reinterpret_cast<const double2*>(Wl + W_PSI_HI * ws);
+#pragma unroll
+    for (int c = 0; c < S; ++c) wh[c] = w[tid + c * WG];
+  }
+  double2 tq[TQ], va[S], vb[S];
+#pragma unroll
+  for (int c = 0; c < TQ; ++c) tq[c] = tw[tid + c * WG];
+  const double2* __restrict__ srcA = reinterpret_cast<const double2*>(re + (long long)y0 * nx);
+  const double2* __restrict__ srcB = reinterpret_cast<const double2*>(re + (long long)(y0 + (two ? 1 : 0)) * nx);
+#pragma unroll
+  for (int c = 0; c < S; ++c) va[c] = srcA[tid + c * WG];
+#pragma unroll
+  for (int c = 0; c < S; ++c) vb[c] = srcB[tid + c * WG];
+
+#pragma unroll
+  for (int c = 0; c < TQ; ++c) twl[tid + c * WG] = tq[c];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1 && !two) break;
+    double2* x = r == 0 ? rowA : rowB;
+    const bool lift = r == 0 ? wlo : (whi && two);
+    const bool lift_hi = r == 0 && whi && !two;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const double2 v = r == 0 ? va[c] : vb[c];
+      double v0 = -pc * v.x, v1 = -pc * v.y;
+      if (r == 0 && lift) {
+        v0 -= wl[c].x;
+        v1 -= wl[c].y;
+      }
+      if ((r == 1 && lift) || lift_hi) {
+        v0 -= wh[c].x;
+        v1 -= wh[c].y;
+      }
+      x[sw(tid + c * WG)] = make_double2(v0, v1);
+    }
+    __syncthreads();
+    stockham<-1, WG, 8>(x, 1, N, lgN, twl, lgnx);
+    // split X[k] = E[k] + W^k O[k], k = 0..N (lanes past N repeat k = N)
+    const int y = y0 + r;
+#pragma unroll
+    for (int c = 0; c <= S; ++c) {
+      const int k = min(tid + c * WG, N);
+      const double2 zk = x[sw(k & (N - 1))];
+      const double2 zc = cconj(x[sw((N - k) & (N - 1))]);
+      const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+      const double2 dd = csub(zk, zc);
+      const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);
+      spec[spec_idx(ny, y, k)] = cadd(E, cmul(tw_half(twl, k, quarter), O));
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv2(StepBuffers b, const int4* __restrict__ tiles,
+                                                          const ChanDesc* __restrict__ chans) {
+  extern __shared__ double2 lds[];
+  constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, quarter = N / 2, half = N / 2;
+  constexpr int TQ = quarter / WG, P = half / WG + 1;  // twiddles, pair slots per lane
+  static_assert(S >= 2 && quarter % WG == 0, "row-pair FFT sizes");
+  const int4 t = tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
+  if (!cd->active) return;
+  const int nx = 2 * N, ny = cd->ny, y0 = t.y;
+  const bool two = t.z == 2;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  const double2* __restrict__ spec = b.spec + cd->spec_off;
+  double* __restrict__ ps = b.psi + cd->fld_off;
+  const int tid = threadIdx.x;
+  double2* rowA = lds;
+  double2* rowB = lds + N;
+  double2* twl = lds + 2 * N;
+  double2 tq[TQ], xa[2][P], xb[2][P];
+#pragma unroll
+  for (int c = 0; c < TQ; ++c) tq[c] = tw[tid + c * WG];
+  // pairs X[p], X[N - p], p = 0..N/2 (lanes past N/2 repeat p = N/2)
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int y = y0 + (r == 1 && two ? 1 : 0);
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int p = min(tid + c * WG, half);
+      xa[r][c] = spec[spec_idx(ny, y, p)];
+      xb[r][c] = spec[spec_idx(ny, y, N - p)];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < TQ; ++c) twl[tid + c * WG] = tq[c];
+  __syncthreads();  // twiddles in LDS before the merge reads them
+  const double invN = 1.0 / (double)N;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1 && !two) break;
+    double2* x = r == 0 ? rowA : rowB;
+    // z[k] = E + i O, E = (X[k] + conj X[N-k]) / 2,
+    // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int p = min(tid + c * WG, half);
+      {
+        const double2 xk = xa[r][c], xc = cconj(xb[r][c]);
+        const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+        const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                               cconj(tw_half(twl, p, quarter)));
+        x[sw(p)] = make_double2(E.x - O.y, E.y + O.x);
+      }
+      if (p > 0 && p < half) {
+        const double2 xk = xb[r][c], xc = cconj(xa[r][c]);
+        const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+        const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                               cconj(tw_half(twl, N - p, quarter)));
+        x[sw(N - p)] = make_double2(E.x - O.y, E.y + O.x);
+      }
+    }
+    __syncthreads();
+    stockham<+1, WG, 8>(x, 1, N, lgN, twl, lgnx);
+    double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx);
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const double2 z = x[sw(tid + c * WG)];
+      dst[tid + c * WG] = make_double2(z.x * invN, z.y * invN);
+    }
+  }
+  // Dirichlet rows psi(0) = voltage, psi(ny-1) = voltage2 (odd_extract,
+  // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
+  const double* __restrict__ Wl = b.walls + cd->wall_off;
+  const long long ws = b.walls_stride;
+  if (y0 == 1) copy_row_batched<WG>(ps, Wl + W_PSI_LO * ws, nx);
+  if (y0 + t.z - 1 == ny - 2) copy_row_batched<WG>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+}
+
+// ---------------------------------------------------------------------------
 // k_tridiag: for each wavenumber k the system
 //   x_{i-1} + a x_i + x_{i+1} = r_i,  i = 1..M,  x_0 = x_{M+1} = 0,
 //   a = -(2 + (kx dy)^2) = -2 cosh(theta)
@@ -1980,7 +2160,11 @@ hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, co
 // The FFT tile table holds the power-of-two tiles first (n_fft_pow2 of them),
 // then the direct-DFT tiles.
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0 && b.fft_persist) {
+  if (b.n_fft_pow2 > 0 && b.fft_pair_s > 0) {
+    const dim3 grid(b.n_fft_pow2), block(FFT_WG_ROWS);
+    if (b.fft_pair_s == 2) hipLaunchKernelGGL(k_fft_fwd2<2>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else hipLaunchKernelGGL(k_fft_fwd2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+  } else if (b.n_fft_pow2 > 0 && b.fft_persist) {
     hipLaunchKernelGGL(k_fft_fwd_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
                        b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {
@@ -1996,7 +2180,11 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
 }
 
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
-  if (b.n_fft_pow2 > 0 && b.fft_persist) {
+  if (b.n_fft_pow2 > 0 && b.fft_pair_s > 0) {
+    const dim3 grid(b.n_fft_pow2), block(FFT_WG_ROWS);
+    if (b.fft_pair_s == 2) hipLaunchKernelGGL(k_fft_inv2<2>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else hipLaunchKernelGGL(k_fft_inv2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+  } else if (b.n_fft_pow2 > 0 && b.fft_persist) {
     hipLaunchKernelGGL(k_fft_inv_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
                        b.n_fft_pow2);
   } else if (b.n_fft_pow2 > 0) {

@@ -77,6 +77,9 @@ _lib_handle = None
 
 # (name, restype, argtypes) for every entry point of include/ekpnp.h
 _P = ctypes.POINTER
+# x-FFT kernel families of a Poisson plan (ekpnp_set_poisson_plan's fft_persistent)
+FFT_KERNELS = ("one-tile", "persistent", "row-pair")
+
 _VP = ctypes.c_void_p
 _DP = _P(ctypes.c_double)
 SIGNATURES = [
@@ -384,20 +387,24 @@ class Solver:
     def load_checkpoint(self, path: str) -> None:
         self._check(lib().ekpnp_load_checkpoint(self._ctx, os.fsencode(path)))
 
-    def set_poisson_plan(self, fft_elems: int = 2048, tri_rows: int = 16, persistent: bool = True) -> None:
-        self._check(lib().ekpnp_set_poisson_plan(self._ctx, int(fft_elems), int(tri_rows), int(bool(persistent))))
+    def set_poisson_plan(self, fft_elems: int = 2048, tri_rows: int = 16, persistent=True) -> None:
+        """persistent: False/0 one-tile x-FFT kernels, True/1 persistent
+        pipelined ones, 2 row-pair ones (channels permitting; otherwise the
+        one-tile kernels)."""
+        self._check(lib().ekpnp_set_poisson_plan(self._ctx, int(fft_elems), int(tri_rows), int(persistent)))
 
     def autotune(self, reps: int = 20) -> dict:
         out = np.zeros(4, dtype=np.float64)
         self._check(lib().ekpnp_autotune(self._ctx, int(reps), _dptr(out)))
         return {"fft_elems": int(out[0]), "tri_rows": int(out[1]), "poisson_ms": float(out[2]),
-                "persistent": bool(out[3])}
+                "persistent": int(out[3]) == 1, "fft_kernels": FFT_KERNELS[int(out[3])]}
 
     def poisson_plan(self) -> dict:
         """The plan in force (ekpnp_get_poisson_plan)."""
         out = (ctypes.c_int * 4)()
         self._check(lib().ekpnp_get_poisson_plan(self._ctx, out))
-        return {"fft_elems": out[0], "tri_rows": out[1], "persistent": bool(out[2]), "walls_in_tri": bool(out[3])}
+        return {"fft_elems": out[0], "tri_rows": out[1], "persistent": out[2] == 1, "fft_kernels": FFT_KERNELS[out[2]],
+                "walls_in_tri": bool(out[3])}
 
     def get_params(self, ch: int) -> ChannelParams:
         out = ChannelParams()

@@ -276,17 +276,21 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
 /* ---- work decomposition ---------------------------------------------------- */
 /* Poisson plan: x-FFT work-groups hold up to fft_elems complex values (whole
  * rows; default 2048), tridiagonal columns are cut into chunks of about
- * tri_rows rows (default 16); fft_persistent != 0 selects the persistent,
- * software-pipelined FFT kernels where the tiles allow them (fft_elems <=
- * 2048, nx <= 4096).  Results do not depend on fft_elems or the kernel
+ * tri_rows rows (default 16); fft_persistent selects the x-FFT kernels:
+ * 0 one tile per work-group, 1 the persistent software-pipelined kernels
+ * where the tiles allow them (fft_elems <= 2048, nx <= 4096), 2 the row-pair
+ * kernels (two rows per work-group transformed one after the other, the
+ * second row's loads in flight during the first's transform) where every
+ * power-of-two channel has nx = 2048 or 4096 and fft_elems = nx; otherwise
+ * the one-tile kernels.  Results do not depend on fft_elems or the kernel
  * choice; tri_rows changes the FP64 rounding of psi only (chunk
  * boundaries). */
 int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent);
 /* Times the Poisson stages over these channels for fft_elems in {1024,
- * 2048, 4096}, both FFT kernel families and tri_rows in {8, 16} (reps solves
+ * 2048, 4096}, the three FFT kernel families and tri_rows in {8, 16} (reps solves
  * each, HIP events on the context's stream), keeps the fastest, and restores
  * psi.  out (may be NULL, else 4 doubles) receives {fft_elems, tri_rows, ms
- * per solve, persistent}.  No counterpart in the reference (cuFFT plans are
+ * per solve, fft_persistent}.  No counterpart in the reference (cuFFT plans are
  * fixed by main.cu:113). */
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
 /* The plan in force: out[0] fft_elems, out[1] tri_rows, out[2] persistent FFT
