@@ -1,9 +1,10 @@
 """The benchmark workloads at their full sizes against the FP64 oracle.
 
-A few coupled steps of the 4096 x 1024 channel (the bench workload) and of
-the 8192 x 2048 channel with zeta(x) walls, FP32 deviation storage with FP64
-arithmetic (the row LBM kernel, the one-tile x-FFTs, the tridiagonal at its
-real column height), compared field by field with the oracle started from the
+A few coupled steps of the 4096 x 1024 channel (the bench workload: row-pair
+x-FFTs) and of the 8192 x 2048 channel with zeta(x) walls (one-tile x-FFTs),
+FP32 deviation storage with FP64 arithmetic (the row LBM kernel, its wall rows
+on the tridiagonal launch, the tridiagonal at its real column height),
+compared field by field with the oracle started from the
 same seeded state; and fluid mass conservation over a few hundred steps of the
 full-size channel, a property that does not depend on the oracle.
 """
@@ -45,6 +46,11 @@ def test_bench_channel_4096x1024_against_the_oracle():
     print("case 4096x1024 tau 0.6 zeta -2 kappa_h 32 E_x 5e-5 seed 1, 12 steps", flush=True)
     p = ekpnp.lattice_params(4096, 1024, tau=0.6, zeta=-2.0, kappa_h=32.0, e_x=5e-5)
     s, o = _run_pair(p, 1, 12)
+    # the default plan for this width: row-pair x-FFTs, wall rows on the
+    # tridiagonal launch
+    plan = s.poisson_plan()
+    print("plan", plan, flush=True)
+    assert plan["fft_kernels"] == "row-pair" and plan["walls_in_tri"]
     errs = field_errors(s, 0, o)
     print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     for f in ("rho", "ux", "cpos", "cneg", "psi"):
