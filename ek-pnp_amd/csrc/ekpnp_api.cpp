@@ -328,8 +328,12 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   c->sb.tri_tiles = c->d_tri_tiles;
   c->sb.n_tri_tiles = (int)c->tri_tiles.size();
   // rows + quadrant twiddle table in LDS when the device allows that much
-  // dynamic LDS per work-group; otherwise twiddles are read from global memory
-  c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)c->max_lds) ? 1 : 0;
+  // dynamic LDS per work-group and the table costs no resident work-group per
+  // CU (nx = 8192: 96 KB with the table, one work-group per CU, 64 KB
+  // without, two); otherwise twiddles are read from global memory (same values)
+  c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)c->max_lds &&
+                      fft_one_tile_blocks_per_cu(true, lds_pow2_tw) >= fft_one_tile_blocks_per_cu(false, lds_pow2))
+                         ? 1 : 0;
   c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
   c->sb.fft_lds_dft = lds_dft;
   c->sb.tri_lmax = tri_lmax;

@@ -124,6 +124,9 @@ hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s);
 // read src and write dst (FP32 storage, FP64 arithmetic)
 hipError_t launch_poisson_tri_walls(bool macro, const StepBuffers& b, const void* src, void* dst, hipStream_t s);
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
+// resident one-tile x-FFT work-groups per CU with twiddles in LDS (twl) or
+// not, lds bytes of dynamic LDS each (registers and LDS both counted); 0 on error
+int fft_one_tile_blocks_per_cu(bool twl, unsigned lds);
 // chunk-entry Thomas coefficients of channel chan for its current tri_C, tri_L
 hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s);
 hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* dst, hipStream_t s);

@@ -2179,6 +2179,13 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
   return hipGetLastError();
 }
 
+int fft_one_tile_blocks_per_cu(bool twl, unsigned lds) {
+  int n = 0;
+  hipError_t e = twl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_fwd<true, true>, FFT_WG_ROWS, lds)
+                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_fwd<true, false>, FFT_WG_ROWS, lds);
+  return e == hipSuccess ? n : 0;
+}
+
 hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_pow2 > 0 && b.fft_pair_s > 0) {
     const dim3 grid(b.n_fft_pow2), block(FFT_WG_ROWS);
