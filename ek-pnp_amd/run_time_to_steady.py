@@ -11,9 +11,11 @@
    default on the difference between those profiles and the physics
    baseline (Poisson-Boltzmann on the channel's nodes + Helmholtz-
    Smoluchowski u_x), so that BF16 only has to resolve a small correction.
-3. Held-out channels (another seed, nx from the config-2 set) are run three
-   times: cold (uniform c0 + noise), warm (predicted profiles broadcast in x)
-   and baseline-warm (the physics baseline alone).
+3. Held-out channels (another seed, nx from the config-2 set) are run four
+   times: cold (uniform c0 + noise), warm (predicted profiles broadcast in x),
+   baseline-warm (the physics baseline alone) and the reference's own start
+   (gpu_initialization with its 501 PB-Picard iterations, LBM.cu:95-160);
+   --arms picks a subset (without "warm" nothing is trained).
    Every warm channel first settles for 100 steps and then runs a probe of
    `probe` steps; a channel whose u_x residual over the probe exceeds
    `accept` (or is not finite) is rejected
@@ -71,6 +73,40 @@ def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold"):
     steps = run_to_residual(s, tol, every, max_steps, tag)
     s.synchronize()
     return s, steps, time.perf_counter() - t0
+
+
+def run_picard(channels, precision, seed, tol, every, max_steps, probe, accept=np.inf, settle=100):
+    """The reference's start: uniform rho0, u = 0, psi and c+- from the
+    PB-Picard iteration on the channel's nodes, equilibrium populations.
+    Such a start has the double layer formed but no flow (u_x still grows
+    linearly over the probe, so the warm arms' residual test would reject
+    every channel) and rho uniform, with no pressure to balance the body
+    force, so strongly charged channels blow up: only channels with
+    non-finite values after settle + probe restart from their cold fields
+    (accept = inf; a finite accept applies the warm arms' test).  Step
+    counts include settle + probe."""
+    t0 = time.perf_counter()
+    s = ekpnp.Solver(channels, precision=precision)
+    s.initialization(ekpnp.COLD_NOISE, seed)
+    cold = [{f: s.get_field(c, f) for f in ("rho", "ux", "uy", "cpos", "cneg")} for c in range(len(channels))]
+    s.initialization(ekpnp.PB_PICARD, seed)
+    s.init_equilibrium()
+    s.step(settle)
+    s.reductions()  # residual snapshot
+    s.step(probe)
+    red = s.reductions()
+    res = red[:, 7]
+    rejected = [c for c in range(len(channels)) if not (np.isfinite(res[c]) and res[c] <= accept)
+                or red[c, 6] > 0]
+    for c in rejected:
+        f = cold[c]
+        s.set_fields(c, f["rho"], f["ux"], f["uy"], f["cpos"], f["cneg"])
+    if rejected:
+        s.init_equilibrium()
+    steps = run_to_residual(s, tol, every, max_steps, "pb-picard")
+    s.synchronize()
+    total = np.where(steps >= 0, steps + settle + probe, -1)
+    return s, total, time.perf_counter() - t0, res, rejected
 
 
 def generate(channels, precision, seed, tol, every, max_steps, residual):
@@ -201,9 +237,14 @@ def main():
                          "--train/--seed/--ny/--nx-train/--targets instead of generating them")
     ap.add_argument("--targets", choices=("direct", "residual"), default="residual",
                     help="learn the profiles themselves or their difference from the physics baseline")
+    ap.add_argument("--arms", default="cold,warm,baseline,picard",
+                    help="held-out arms to run (cold is always run: the others are measured against it)")
     args = ap.parse_args()
     ny = tuple(int(v) for v in args.ny.split(","))
     nxs = tuple(int(v) for v in args.nx.split(","))
+    arms = set(args.arms.split(","))
+    if "warm" not in arms:
+        return held_out_only(args, ny, nxs, arms)
 
     tr_ch, rej_tr = S.sample_channels(args.train, args.seed, nx_choices=(args.nx_train,), ny_choices=ny)
     log(f"training channels: {len(tr_ch)} (redrawn {rej_tr})")
@@ -239,6 +280,14 @@ def main():
                                                       args.probe, args.accept)
     s.close()
     log(f"physics baseline: median steps {np.median(base):.0f}, wall {base_wall:.1f} s, rejected {len(base_rej)}")
+    picard = None
+    if "picard" in arms:
+        s, pic, pic_wall, pic_res, pic_rej = run_picard(ho_ch, args.precision, 900000 + args.seed, args.tol,
+                                                        args.every, args.max_steps, args.probe)
+        s.close()
+        log(f"pb-picard: median steps {np.median(pic):.0f}, wall {pic_wall:.1f} s, rejected {len(pic_rej)}")
+        picard = {"steps": pic.tolist(), "rejected": pic_rej, "wall_s": round(pic_wall, 2),
+                  "probe_residuals": [float(v) for v in pic_res], "summary": summarize(cold, pic)}
     out = {
         "experiment": "time-to-steady, surrogate-warm vs cold",
         "tol": args.tol, "check_every": args.every, "precision": args.precision,
@@ -253,11 +302,42 @@ def main():
                     "summary": summarize(cold, warm),
                     "baseline_steps": base.tolist(), "baseline_rejected": base_rej,
                     "baseline_probe_residual_median": float(np.median(base_res)),
-                    "baseline_summary": summarize(cold, base)},
+                    "baseline_summary": summarize(cold, base), "pb_picard": picard},
         "targets": args.targets,
     }
     if args.save:
         np.savez(args.save, params=net.get_params(), X=X, T=T, steps=tr_steps)
+    print(json.dumps(out), flush=True)
+
+
+def held_out_only(args, ny, nxs, arms):
+    """The held-out set alone: cold, optionally the physics baseline and the
+    reference's PB-Picard start, each against cold."""
+    ho_ch, rej_ho = S.sample_channels(args.heldout, 900000 + args.seed, nx_choices=nxs, ny_choices=ny)
+    seed = 900000 + args.seed
+    s, cold, cold_wall = run_cold(ho_ch, args.precision, seed, args.tol, args.every, args.max_steps)
+    s.close()
+    log(f"cold: median steps {np.median(cold):.0f}, wall {cold_wall:.1f} s")
+    out = {"experiment": "time-to-steady on the held-out channels, other starts vs cold", "tol": args.tol,
+           "check_every": args.every, "precision": args.precision,
+           "heldout": {"channels": len(ho_ch), "redrawn": rej_ho, "nx": list(nxs), "ny": list(ny),
+                       "cold_steps": cold.tolist(), "cold_wall_s": round(cold_wall, 2)}}
+    if "baseline" in arms:
+        s, base, base_wall, base_res, base_rej = run_warm(ho_ch, predictor(None, True), args.precision, seed,
+                                                          args.tol, args.every, args.max_steps, args.probe,
+                                                          args.accept)
+        s.close()
+        log(f"physics baseline: median steps {np.median(base):.0f}, wall {base_wall:.1f} s")
+        out["heldout"]["baseline"] = {"steps": base.tolist(), "rejected": base_rej, "wall_s": round(base_wall, 2),
+                                      "summary": summarize(cold, base)}
+    if "picard" in arms:
+        s, pic, pic_wall, pic_res, pic_rej = run_picard(ho_ch, args.precision, seed, args.tol, args.every,
+                                                        args.max_steps, args.probe)
+        s.close()
+        log(f"pb-picard: median steps {np.median(pic):.0f}, wall {pic_wall:.1f} s, rejected {len(pic_rej)}")
+        out["heldout"]["pb_picard"] = {"steps": pic.tolist(), "rejected": pic_rej, "wall_s": round(pic_wall, 2),
+                                       "probe_residuals": [float(v) for v in pic_res],
+                                       "summary": summarize(cold, pic)}
     print(json.dumps(out), flush=True)
 
 
