@@ -1047,28 +1047,42 @@ int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
   if (nsteps == 0) return EKPNP_OK;
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
-  int n = nsteps;
-  if (ctx->state_q) {
-    HIP_OR_FAIL(ctx, enqueue_steps(ctx, 1, n == 1));
-    n -= 1;
-  }
-  if (!ctx->opt.use_graph) {
-    HIP_OR_FAIL(ctx, enqueue_steps(ctx, n, true));
-  } else {
-    const int CHUNK = 64;
-    while (n > 0) {
-      const int k = n > CHUNK ? CHUNK : n;
-      const bool last = (k == n);
-      int rc = graph_steps(ctx, k, last);
+  // FP32 storage follows the drifting row means so that the stored
+  // deviations stay small: a long call is cut into segments that end where
+  // a re-centring is due (the last step of each segment writes the
+  // macroscopic planes the re-centring reads)
+  const bool fp32 = ctx->opt.precision == EKPNP_POP_FP32;
+  int left = nsteps;
+  while (left > 0) {
+    int n = left;
+    if (fp32) {
+      const long long until = EKPNP_REBASE_INTERVAL - (ctx->steps - ctx->rebase_step);
+      if (until >= 1 && until < n) n = (int)until;
+    }
+    const int seg = n;
+    if (ctx->state_q) {
+      HIP_OR_FAIL(ctx, enqueue_steps(ctx, 1, n == 1));
+      n -= 1;
+    }
+    if (!ctx->opt.use_graph) {
+      HIP_OR_FAIL(ctx, enqueue_steps(ctx, n, true));
+    } else {
+      const int CHUNK = 64;
+      while (n > 0) {
+        const int k = n > CHUNK ? CHUNK : n;
+        const bool last = (k == n);
+        int rc = graph_steps(ctx, k, last);
+        if (rc) return rc;
+        n -= k;
+      }
+    }
+    ctx->steps += seg;
+    left -= seg;
+    if (fp32 && ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL) {
+      int rc = ekpnp_rebase(ctx);
       if (rc) return rc;
-      n -= k;
     }
   }
-  ctx->steps += nsteps;
-  // FP32 storage follows the drifting row means so that the stored
-  // deviations stay small
-  if (ctx->opt.precision == EKPNP_POP_FP32 && ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL)
-    return ekpnp_rebase(ctx);
   return EKPNP_OK;
 }
 

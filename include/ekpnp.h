@@ -220,7 +220,7 @@ int ekpnp_fast_poisson(ekpnp_ctx* ctx);
 int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double tol, int max_iter,
                             int check_every, int* iters_out, double* change_out);
 /* With FP32 storage, ekpnp_step re-centres the storage reference every
- * EKPNP_REBASE_INTERVAL steps (see ekpnp_rebase). */
+ * EKPNP_REBASE_INTERVAL steps (see ekpnp_rebase), inside a long call too. */
 int ekpnp_step(ekpnp_ctx* ctx, int nsteps);
 /* FP32 storage keeps each population as a deviation from w_i times a per-row
  * reference (rho, c+, c-)(y).  This re-centres the references on the current

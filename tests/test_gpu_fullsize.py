@@ -86,3 +86,25 @@ def test_bench_channel_conserves_fluid_mass():
     print("mass", m0, m1, (m1 - m0) / m0, flush=True)
     assert red[ekpnp.RED_NAMES.index("nonfinite")] == 0
     assert abs(m1 - m0) <= 1e-7 * m0
+
+
+def test_bench_channel_fp32_tracks_fp64_storage_across_a_recentring():
+    # 6000 steps in one call (the storage reference is re-centred at step
+    # 4096 inside it): FP32 deviation storage against FP64 storage, the path
+    # that matches the oracle to rounding, within the 1e-5 relative-L2 bar
+    print("case 4096x1024 fp32 vs fp64 storage, 6000 steps", flush=True)
+    p = ekpnp.lattice_params(4096, 1024, tau=0.6, zeta=-2.0, kappa_h=32.0, e_x=5e-5)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s = ekpnp.Solver([p], precision=prec)
+        s.initialization(ekpnp.COLD_NOISE, 1)
+        s.init_equilibrium()
+        s.step(6000)
+        out[prec] = {f: s.get_field(0, f) for f in ("ux", "psi", "cpos", "cneg")}
+        s.close()
+    errs = {f: float(np.linalg.norm(out["fp32"][f] - out["fp64"][f]) / np.linalg.norm(out["fp64"][f]))
+            for f in out["fp64"]}
+    print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
+    for f, e in errs.items():
+        assert e < 1e-5, (f, e)
+

@@ -303,6 +303,24 @@ def test_wall_rows_on_the_tridiagonal_launch_are_bit_identical(monkeypatch):
         np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=str(k))
 
 
+def test_recentring_inside_a_long_step_call():
+    # FP32 storage re-centres every 4096 steps, also inside one long call:
+    # step(5000) and step(4096) + step(904) re-centre at the same step and
+    # give identical bits
+    p = ekpnp.lattice_params(256, 40, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4)
+    runs = []
+    for split in ((5000,), (4096, 904)):
+        s = ekpnp.Solver([p], precision="fp32")
+        s.initialization(ekpnp.COLD_NOISE, 6)
+        s.init_equilibrium()
+        for n in split:
+            s.step(n)
+        runs.append({sp: s.get_populations(0, sp) for sp in ("f", "h", "g")} | {"psi": s.get_field(0, "psi")})
+        s.close()
+    for k in runs[0]:
+        np.testing.assert_array_equal(runs[0][k], runs[1][k], err_msg=k)
+
+
 def test_fp32_row_kernel_channel_against_the_oracle():
     # a 256-wide FP32 channel (row kernel) and a 96-wide one (k_lbm) in one
     # batch, each against its FP64 oracle after 2000 steps
