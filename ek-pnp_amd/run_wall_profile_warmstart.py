@@ -17,6 +17,12 @@ residual and the decision, the residual after the run, and the relative L2
 distance of the x-averaged psi, c+, c-, u_x from the cold arm's; the cold arm
 reports its own residual over the same probe window.
 
+The "-local" arms predict psi, c+-, rho column by column from the local wall
+potential zeta(x) (profiles at 65 levels spanning zeta(x)'s range, linearly
+interpolated), so the double layer follows the wall charge; u_x stays the
+profile of the mean zeta broadcast in x (an x-varying u_x would violate
+continuity).  --arms picks the arms; cold always runs.
+
 Prints one JSON line.
 """
 import argparse
@@ -33,14 +39,37 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 import ekpnp  # noqa: E402
 from ekpnp import surrogate as S  # noqa: E402
-from bench import apply_walls, workload  # noqa: E402
+from bench import apply_walls, wall_zeta, workload  # noqa: E402
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_arm(params, predict, steps, settle, probe, accept, tag):
+def column_fields(p, predict, zeta, levels=65):
+    """(rho, ux, c+, c-, psi) as [ny][nx] arrays: psi, c+-, rho per column from
+    profiles predicted at `levels` wall potentials spanning zeta's range,
+    linearly interpolated to each column's zeta(x); u_x from the mean-zeta
+    profile."""
+    zl = np.linspace(zeta.min(), zeta.max(), levels)
+    prof = []
+    for z in zl:
+        q = p.copy()
+        q.psi_lo = q.psi_hi = float(z)
+        q.cpos_lo = q.cpos_hi = float(np.exp(-z))
+        q.cneg_lo = q.cneg_hi = float(np.exp(z))
+        psi, cp, cn, _ = predict(q)
+        prof.append(np.stack([S.osmotic_rho(q, psi, cp, cn), cp, cn, psi]))
+    prof = np.stack(prof)                                  # [levels][4][ny]
+    t = (zeta - zl[0]) / (zl[1] - zl[0])
+    i = np.clip(np.floor(t).astype(int), 0, levels - 2)
+    w = (t - i)[None, None, :]
+    cols = (1.0 - w) * prof[i].transpose(1, 2, 0) + w * prof[i + 1].transpose(1, 2, 0)  # [4][ny][nx]
+    _, _, _, ux = predict(p)
+    return cols[0], np.repeat(ux[:, None], p.nx, axis=1), cols[1], cols[2], cols[3]
+
+
+def run_arm(params, predict, steps, settle, probe, accept, tag, local=False):
     s = ekpnp.Solver(params, precision="fp32")
     apply_walls(s, "c4", params, 4)
     s.initialization(ekpnp.COLD_NOISE, 4)
@@ -49,9 +78,14 @@ def run_arm(params, predict, steps, settle, probe, accept, tag):
     if predict is not None:
         p = params[0]
         cold = {f: s.get_field(0, f) for f in ("rho", "ux", "uy", "cpos", "cneg")}
-        psi, cp, cn, ux = predict(p)
-        b = lambda v: np.repeat(v[:, None], p.nx, axis=1)  # noqa: E731
-        s.set_fields(0, b(S.osmotic_rho(p, psi, cp, cn)), b(ux), np.zeros((p.ny, p.nx)), b(cp), b(cn), b(psi))
+        if local:
+            rho2, ux2, cp2, cn2, psi2 = column_fields(p, predict, wall_zeta(p.nx, 4))
+            s.set_fields(0, rho2, ux2, np.zeros((p.ny, p.nx)), cp2, cn2, psi2)
+        else:
+            psi, cp, cn, ux = predict(p)
+            b = lambda v: np.repeat(v[:, None], p.nx, axis=1)  # noqa: E731
+            s.set_fields(0, b(S.osmotic_rho(p, psi, cp, cn)), b(ux), np.zeros((p.ny, p.nx)), b(cp), b(cn),
+                         b(psi))
         s.init_equilibrium()
         s.step(settle)
         s.reductions()
@@ -99,7 +133,11 @@ def main():
     ap.add_argument("--settle", type=int, default=100)
     ap.add_argument("--probe", type=int, default=200)
     ap.add_argument("--accept", type=float, default=1e-4)
+    ap.add_argument("--arms", default="surrogate,physics baseline",
+                    help="comma-separated among: surrogate, physics baseline, surrogate-local, "
+                         "physics baseline-local")
     args = ap.parse_args()
+    arms = args.arms.split(",")
     params, _, _, desc = workload("c4")
     saved = np.load(args.net)
     net = S.MLP(max_batch=8)
@@ -112,8 +150,11 @@ def main():
     cold_info, cold_prof = run_arm(params, None, args.steps, args.settle, args.probe, args.accept, "cold")
     out = {"experiment": "surrogate warm start on the heterogeneous-wall-charge channel", "workload": desc,
            "steps": args.steps, "accept": args.accept, "arms": [cold_info]}
-    for tag, pred in (("surrogate", surrogate), ("physics baseline", S.baseline)):
-        info, prof = run_arm(params, pred, args.steps, args.settle, args.probe, args.accept, tag)
+    preds = {"surrogate": surrogate, "physics baseline": S.baseline}
+    for tag in arms:
+        local = tag.endswith("-local")
+        pred = preds[tag[:-len("-local")] if local else tag]
+        info, prof = run_arm(params, pred, args.steps, args.settle, args.probe, args.accept, tag, local)
         info["rel_l2_vs_cold"] = {f: float(np.linalg.norm(prof[f] - cold_prof[f]) / np.linalg.norm(cold_prof[f]))
                                   for f in prof}
         out["arms"].append(info)
