@@ -96,3 +96,31 @@ def test_osmotic_density_matches_the_oracle_steady_state():
     dev = rho - rho.mean()
     assert np.linalg.norm((pred - pred.mean()) - dev) < 0.02 * np.linalg.norm(dev)
     assert np.linalg.norm(pred - rho) < 0.1 * np.linalg.norm(dev)
+
+
+def test_column_fields_follow_the_local_wall_potential():
+    # the column-local warm start of the heterogeneous-wall channel: psi, c+-
+    # and rho interpolated per column between profiles at wall-potential
+    # levels; at a level's own zeta the column is that level's profile
+    # exactly, the wall row carries zeta(x) itself (psi is linear in the
+    # level there), and u_x is the mean-zeta profile in every column
+    import run_wall_profile_warmstart as R
+
+    p = S.channel_of(-1.5, 8.0, 1e-4, 16, 40)
+    zeta = np.linspace(-2.0, -1.0, 16)
+    rho, ux, cp, cn, psi = R.column_fields(p, S.baseline, zeta, levels=16)
+    assert rho.shape == ux.shape == cp.shape == cn.shape == psi.shape == (40, 16)
+    np.testing.assert_allclose(psi[0], zeta, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(psi[-1], zeta, rtol=0, atol=1e-14)
+    for x in (0, 7, 15):
+        q = p.copy()
+        q.psi_lo = q.psi_hi = float(zeta[x])
+        q.cpos_lo = q.cpos_hi = float(np.exp(-zeta[x]))
+        q.cneg_lo = q.cneg_hi = float(np.exp(zeta[x]))
+        bpsi, bcp, bcn, _ = S.baseline(q)
+        np.testing.assert_allclose(psi[:, x], bpsi, rtol=1e-12, atol=1e-14)
+        np.testing.assert_allclose(cp[:, x], bcp, rtol=1e-12)
+        np.testing.assert_allclose(rho[:, x], S.osmotic_rho(q, bpsi, bcp, bcn), rtol=1e-12)
+    u_mean = S.baseline(p)[3]
+    for x in range(16):
+        np.testing.assert_array_equal(ux[:, x], u_mean)
