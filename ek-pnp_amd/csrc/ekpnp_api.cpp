@@ -379,7 +379,7 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   {
     const char* env = std::getenv("EKPNP_WALLS_IN_TRI");
     const long long slots = 3LL * c->n_cu;
-    const long long nt = (long long)c->tri_tiles.size(), nw = c->n_lbm_wall;
+    const long long nt = (long long)c->tri_tiles.size(), nw = (c->n_lbm_wall + 7) & ~7;
     c->walls_in_tri = !(env && env[0] == '0') && nw > 0 && (nt + nw + slots - 1) / slots == (nt + slots - 1) / slots;
   }
   c->fft_elems = fft_elems;

@@ -1829,8 +1829,11 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
 }
 
 // The tridiagonal launch of a coupled step with the wall rows of the step's
-// collide-stream appended: work-groups past the tridiagonal tiles update the
-// row-kernel wall tiles (as k_lbm_rows<MACRO, true>).  Valid because the
+// collide-stream added: the first nw work-groups (the wall tiles, padded to a
+// multiple of 8 so that the tridiagonal tiles keep their XCD placement) update
+// the row-kernel wall tiles (as k_lbm_rows<MACRO, true>), the rest solve the
+// tridiagonal tiles.  Wall work-groups go first so that they start with the
+// first round of tridiagonal tiles instead of waiting for a late slot.  Valid because the
 // wall-row update reads only the step's source populations, psi and the
 // storage references, none of which the forward x-FFT or the tridiagonal
 // writes, and writes rows 0 and ny-1, which neither of them reads; the wall
@@ -1842,11 +1845,12 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag_walls(StepBuffers b, const f
                                                           float* __restrict__ dst) {
   __shared__ double sA[TRI_WG];
   __shared__ double2 sB[TRI_WG];
-  const int nt = b.n_tri_tiles;
-  if ((int)blockIdx.x < nt)
-    tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(blockIdx.x, nt)], sA, sB);
-  else
-    lbm_rows_tile<MACRO, true>(b, b.lbm_wall_tiles[blockIdx.x - nt], src, dst);
+  const int nt = b.n_tri_tiles, nw = (b.n_lbm_wall_tiles + 7) & ~7;
+  const int i = blockIdx.x;
+  if (i >= nw)
+    tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(i - nw, nt)], sA, sB);
+  else if (i < b.n_lbm_wall_tiles)
+    lbm_rows_tile<MACRO, true>(b, b.lbm_wall_tiles[i], src, dst);
 }
 
 // table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
@@ -2220,7 +2224,7 @@ hipError_t launch_poisson_tri(const StepBuffers& b, hipStream_t s) {
 hipError_t launch_poisson_tri_walls(bool macro, const StepBuffers& b, const void* src, void* dst, hipStream_t s) {
   const float* fs = static_cast<const float*>(src);
   float* fd = static_cast<float*>(dst);
-  const dim3 grid(b.n_tri_tiles + b.n_lbm_wall_tiles);
+  const dim3 grid(b.n_tri_tiles + ((b.n_lbm_wall_tiles + 7) & ~7));
   if (macro) hipLaunchKernelGGL((k_tridiag_walls<TRI_LMAX, true>), grid, dim3(TRI_WG), 0, s, b, fs, fd);
   else hipLaunchKernelGGL((k_tridiag_walls<TRI_LMAX, false>), grid, dim3(TRI_WG), 0, s, b, fs, fd);
   return hipGetLastError();
