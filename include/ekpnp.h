@@ -288,7 +288,8 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
 int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent);
 /* Times the Poisson stages over these channels for fft_elems in {1024,
  * 2048, 4096}, the three FFT kernel families and tri_rows in {8, 16} (reps solves
- * each, HIP events on the context's stream), keeps the fastest, and restores
+ * each, three passes over the candidates, HIP events on the context's
+ * stream), keeps the fastest, and restores
  * psi.  out (may be NULL, else 4 doubles) receives {fft_elems, tri_rows, ms
  * per solve, fft_persistent}.  No counterpart in the reference (cuFFT plans are
  * fixed by main.cu:113). */
@@ -312,7 +313,8 @@ long long ekpnp_steps_done(const ekpnp_ctx* ctx);
 size_t ekpnp_device_bytes(const ekpnp_ctx* ctx);
 /* Times nsteps eager steps with HIP events on the context's stream, per
  * stage: ms_out[0] LBM, [1] forward x-FFT, [2] wall-normal tridiagonal,
- * [3] inverse x-FFT, [4] whole step (graph replay of nsteps, per step).  The
+ * [3] inverse x-FFT, [4] whole step (graph replay of nsteps after one
+ * untimed replay, per step).  The
  * launches are those of the coupled step: when the plan puts the wall rows
  * on the tridiagonal launch (ekpnp_get_poisson_plan), [0] covers the other
  * rows and [2] includes the wall rows. */
