@@ -67,7 +67,8 @@ def wall_zeta(nx, seed, zeta0=-1.5, modes=4, amp=0.5):
 
 
 def apply_walls(s, name, params, seed):
-    """Per-x wall potential and Boltzmann wall concentrations (config 4)."""
+    """Per-x wall potential and Boltzmann wall concentrations (the 8192 x 2048
+    heterogeneous-wall channel)."""
     import numpy as np
 
     if name != "c4":

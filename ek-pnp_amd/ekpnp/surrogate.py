@@ -339,7 +339,8 @@ def stable(p: ChannelParams, max_kappa_dy=0.4, max_drift=0.05) -> bool:
 
 def sample_channels(n: int, seed: int, nx_choices=(32, 64, 128, 256), ny_choices=None,
                     zeta_range=(-4.0, -0.4), kappa_h_range=(1.0, 50.0), e_x_range=(1e-5, 1e-3)):
-    """Config-2-style channels drawn from a seeded stream; draws that fail
+    """Batched-sweep channels (nx, ny, zeta, kappa h, E_x from the ranges
+    below) drawn from a seeded stream; draws that fail
     stable() are redrawn.  Returns (list of ChannelParams, number rejected)."""
     if ny_choices is None:
         ny_choices = tuple(range(64, 1025, 64))

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Time-to-steady with surrogate warm starts versus cold starts.
 
-1. Training data: seeded config-2-style channels (zeta ~ U[-4,-0.4], kappa h ~
+1. Training data: seeded batched-sweep channels (zeta ~ U[-4,-0.4], kappa h ~
    logU[1,50], E_x ~ logU[1e-5,1e-3], draws failing the resolution/stability
    rule redrawn) are run by the batched solver from a cold start to the
    u_x residual tolerance; their x-averaged steady profiles of psi, c+, c-,
@@ -11,7 +11,7 @@
    default on the difference between those profiles and the physics
    baseline (Poisson-Boltzmann on the channel's nodes + Helmholtz-
    Smoluchowski u_x), so that BF16 only has to resolve a small correction.
-3. Held-out channels (another seed, nx from the config-2 set) are run four
+3. Held-out channels (another seed, nx from {32, 64, 128, 256}) are run four
    times: cold (uniform c0 + noise), warm (predicted profiles broadcast in x),
    baseline-warm (the physics baseline alone) and the reference's own start
    (gpu_initialization with its 501 PB-Picard iterations, LBM.cu:95-160);
