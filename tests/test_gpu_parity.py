@@ -37,8 +37,8 @@ def test_initial_fields_match():
 
 
 @pytest.mark.parametrize("nsteps", [1, 10, 100, 1000])
-def test_fp64_config0_steps(nsteps):
-    # config 0: 64 x 128, tau 0.8, zeta -1, kappa h 8, E_x 1e-4, seed 0
+def test_fp64_64x128_channel_steps(nsteps):
+    # the 64 x 128 channel: tau 0.8, zeta -1, kappa h 8, E_x 1e-4, seed 0
     p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
     s, o = _pair(p)
     s.step(nsteps)
@@ -209,7 +209,7 @@ def test_batched_channels_match_single():
         assert_parity(s, ch, o, FP64_TOL, tag=f"batched ch{ch}")
 
 
-def _fp32_errors_config0(compute64):
+def _fp32_errors_64x128(compute64):
     p = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
     o = O.Oracle(p)
     o.initialization(ekpnp.COLD_NOISE, 0, 0)
@@ -225,10 +225,10 @@ def _fp32_errors_config0(compute64):
     return errs
 
 
-def test_fp32_deviation_storage_config0():
+def test_fp32_deviation_storage_64x128():
     # FP32 storage with FP64 collision arithmetic (the default): the 1e-5
     # relative-L2 bar after 20000 steps (past the 3 rebases it triggers)
-    errs = _fp32_errors_config0(True)
+    errs = _fp32_errors_64x128(True)
     for f in ("ux", "cpos", "cneg", "psi"):
         assert errs[f] < 1e-5, (f, errs[f])
 
@@ -237,7 +237,7 @@ def test_fp32_arithmetic_variant_is_bounded():
     # FP32 collision arithmetic (compute_fp64=False) is an option, not the
     # default: its u_x error grows to ~1e-4 over 20000 steps (the small
     # drift velocity is a difference of O(1) population sums), bounded here
-    errs = _fp32_errors_config0(False)
+    errs = _fp32_errors_64x128(False)
     for f in ("cpos", "cneg"):
         assert errs[f] < 1e-5, (f, errs[f])
     assert errs["psi"] < 1e-4
@@ -444,7 +444,7 @@ def test_run_to_residual_freezes_converged_channels_exactly():
 
 
 def test_heterogeneous_wall_charge_parity():
-    # config-4 physics on a small grid: zeta(x) with 4 Fourier modes on both
+    # heterogeneous-wall physics on a small grid: zeta(x) with 4 Fourier modes on both
     # walls, Boltzmann wall concentrations, per-x lifts in the Poisson solve
     nx, ny = 64, 48
     p = ekpnp.lattice_params(nx, ny, tau=0.8, zeta=-1.5, kappa_h=6.0, e_x=1e-4)
