@@ -1056,8 +1056,14 @@ int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
   while (left > 0) {
     int n = left;
     if (fp32) {
+      // overdue (steps taken by other entry points): re-centre first; the
+      // change of representation is exact whatever the references
+      if (ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL) {
+        int rc = ekpnp_rebase(ctx);
+        if (rc) return rc;
+      }
       const long long until = EKPNP_REBASE_INTERVAL - (ctx->steps - ctx->rebase_step);
-      if (until >= 1 && until < n) n = (int)until;
+      if (until < n) n = (int)until;
     }
     const int seg = n;
     if (ctx->state_q) {
