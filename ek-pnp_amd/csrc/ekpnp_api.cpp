@@ -1373,7 +1373,7 @@ int ekpnp_get_poisson_plan(const ekpnp_ctx* ctx, int* out) {
   out[0] = ctx->fft_elems;
   out[1] = ctx->tri_rows;
   out[2] = ctx->fft_persist;
-  out[3] = ctx->walls_in_tri ? 1 : 0;
+  out[3] = (ctx->walls_in_tri && ctx->opt.precision == 0 && ctx->opt.compute_fp64 != 0) ? 1 : 0;
   return EKPNP_OK;
 }
 

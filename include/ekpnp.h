@@ -296,8 +296,9 @@ int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
 /* The plan in force: out[0] fft_elems, out[1] tri_rows, out[2] persistent FFT
  * kernels in use, out[3] whether a coupled step updates the collide-stream's
- * wall rows (the rows y = 0 and ny - 1 of channels with nx a multiple of 64)
- * inside its tridiagonal launch instead of a launch of their own.  That
+ * wall rows (the rows y = 0 and ny - 1 of channels with nx a multiple of 64,
+ * FP32 storage with FP64 arithmetic) inside its tridiagonal launch instead of
+ * a launch of their own.  That
  * placement is chosen with the plan, when the wall work-groups fit beside
  * the tridiagonal's without another round; EKPNP_WALLS_IN_TRI=0 in the
  * environment at plan time turns it off.  Results do not depend on it.  No

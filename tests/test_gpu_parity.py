@@ -172,6 +172,17 @@ def test_row_pair_fft_kernels(nx, ny):
     assert rel_l2(s.get_field(0, "psi"), want) < 1e-12
 
 
+def test_plan_arguments_are_validated():
+    s = ekpnp.Solver([ekpnp.lattice_params(64, 20)], precision="fp64")
+    for args in ((0, 16, 0), (8192, 16, 0), (2048, 0, 0), (2048, 17, 0), (2048, 16, 3), (2048, 16, -1)):
+        with pytest.raises(ekpnp.EkpnpError):
+            s.set_poisson_plan(*args)
+    s.set_poisson_plan(2048, 8, 1)
+    plan = s.poisson_plan()
+    assert plan["fft_elems"] == 2048 and plan["tri_rows"] == 8 and plan["fft_kernels"] == "persistent"
+    assert not plan["walls_in_tri"]  # FP64 storage: no row kernel
+
+
 def test_row_pair_fft_kernels_in_a_coupled_batch():
     # two 2048-wide channels on the row-pair kernels beside a 10-wide one on
     # the direct DFT, FP64 storage, against the oracle after 40 steps
