@@ -203,6 +203,8 @@ def main():
     st = s.time_stages(args.stage_steps)
     red = s.reductions()
     walls_in_tri = s.poisson_plan()["walls_in_tri"]
+    if plan is not None:
+        plan["walls_in_tri"] = walls_in_tri
     s.close()
     lbm_b, coupled_b = alg_bytes(precision)
     if walls_in_tri:
