@@ -218,7 +218,9 @@ def main():
     ap.add_argument("--train", type=int, default=8192)
     ap.add_argument("--heldout", type=int, default=64)
     ap.add_argument("--ny", default="32,48,64,96,128")
-    ap.add_argument("--nx-train", type=int, default=4)
+    ap.add_argument("--nx-train", type=int, default=4,
+                    help="width of the training channels (the steady state does not depend on it); 8 runs "
+                         "the x-FFT on many-row tiles and is much faster for large batches of tall channels")
     ap.add_argument("--nx", default="32,64")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--every", type=int, default=100)
