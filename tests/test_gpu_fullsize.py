@@ -108,3 +108,29 @@ def test_bench_channel_fp32_tracks_fp64_storage_across_a_recentring():
     for f, e in errs.items():
         assert e < 1e-5, (f, e)
 
+
+
+def test_batched_sweep_fp32_tracks_fp64_storage():
+    # the 256-channel batched sweep (seed 2: nx in {32, 64, 128, 256}, ny in
+    # 64..1024, mixed zeta, kappa h, E_x; row kernel and k_lbm channels, wall
+    # rows on the tridiagonal launch) after 5000 steps: FP32 deviation
+    # storage against FP64 storage, every channel within the 1e-5 bar
+    from ekpnp import surrogate as S
+
+    print("case batched sweep, 256 channels seed 2, 5000 steps", flush=True)
+    ps, _ = S.sample_channels(256, 2)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        s = ekpnp.Solver(ps, precision=prec)
+        s.initialization(ekpnp.COLD_NOISE, 2)
+        s.init_equilibrium()
+        s.step(5000)
+        out[prec] = [{f: s.get_field(c, f) for f in ("ux", "psi", "cpos", "cneg")} for c in range(len(ps))]
+        s.close()
+    worst = {f: 0.0 for f in ("ux", "psi", "cpos", "cneg")}
+    for a, b in zip(out["fp32"], out["fp64"]):
+        for f in worst:
+            worst[f] = max(worst[f], float(np.linalg.norm(a[f] - b[f]) / np.linalg.norm(b[f])))
+    print(" ".join(f"{k}={v:.2e}" for k, v in worst.items()), flush=True)
+    for f, e in worst.items():
+        assert e < 1e-5, (f, e)
