@@ -27,12 +27,13 @@ namespace {
 const double W[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
 const int CXH[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
-// storage references: one record of 8 doubles per row, X (rho, c+, c-),
-// J (their x-fluxes), 2 unused
+// storage references: one record of 8 doubles per row, X (rho, c+, c-) row
+// means, J (their x-fluxes), 2 unused; and per row nb block records of
+// BLK_STRIDE floats, X over each 64-node block of the row
 constexpr int kRefStride = 8;
 constexpr int kMaxTriChunks = TRI_WG / 2;  // the plan keeps G = TRI_WG / C >= 2 modes per work-group
 
-// stored value + stored_ref(y, d) = population (FP32 storage; 0 for FP64)
+// stored value + ref[d][y][x] = population (FP32 storage; 0 for FP64)
 std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err);
 
 thread_local std::string g_create_error;
@@ -58,7 +59,7 @@ struct ekpnp_ctx {
   int elem = 4;  // bytes per population value
   std::vector<ChanDesc> desc;
   std::vector<ChannelTiles> ctiles;
-  long long total_nodes = 0, total_spec = 0, total_pops = 0, total_refs = 0;
+  long long total_nodes = 0, total_spec = 0, total_pops = 0, total_refs = 0, total_blocks = 0;
   int total_nx = 0;
   // host copies
   std::vector<int2> lbm_tiles;
@@ -89,6 +90,8 @@ struct ekpnp_ctx {
   double* d_ux_prev = nullptr;
   double* d_refs = nullptr;      // storage references in use, [ny][8] per channel
   double* d_refs_new = nullptr;  // staging for a change of reference
+  float* d_blocks = nullptr;     // block references in use, [ny][nb][BLK_STRIDE] per channel
+  float* d_blocks_new = nullptr; // staging for a change of reference
   int2* d_row_tiles = nullptr;
   double* d_partials = nullptr;
   double* d_red = nullptr;
@@ -105,6 +108,7 @@ struct ekpnp_ctx {
   long long steps = 0;
   long long snapshot_step = -1;
   long long rebase_step = 0;  // step count at the last change of storage reference
+  long long origin_step = 0;  // step count when init_equilibrium last set the populations
   // Poisson work decomposition (see build_poisson_plan)
   int fft_elems = 2048, tri_rows = 16, fft_persist = 1;
   int n_cu = 256;
@@ -149,6 +153,21 @@ std::string validate(const ekpnp_channel_params& p, int ch) {
   return std::string();
 }
 
+// EKPNP_CHECK_* flags of one channel (ekpnp_check_params)
+uint32_t check_flags(const ekpnp_channel_params& p, double* kappa_dy, double* drift) {
+  const double kappa = std::sqrt(std::max(0.0, 2.0 * p.poisson_coef * p.c_bulk * p.inv_thermal_voltage));
+  const double zw = std::max(std::fabs(p.psi_lo), std::fabs(p.psi_hi));
+  const double kmax = std::max(std::fabs(p.mob_pos), std::fabs(p.mob_neg));
+  const double ivt = p.inv_thermal_voltage;
+  const double ew = ivt > 0.0 ? 2.0 * kappa / ivt * std::sinh(0.5 * ivt * zw) : 0.0;  // wall field
+  *kappa_dy = kappa * p.dy;
+  *drift = kmax * ew * p.dt / p.dx;
+  uint32_t f = 0;
+  if (!(*kappa_dy <= 0.4)) f |= EKPNP_CHECK_DEBYE_UNRESOLVED;
+  if (!(*drift <= 0.05)) f |= EKPNP_CHECK_WALL_DRIFT;
+  return f;
+}
+
 ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   ChanDesc d{};
   d.nx = p.nx;
@@ -159,6 +178,7 @@ ChanDesc make_desc(const ekpnp_channel_params& p, int precision) {
   d.swap = 0;
   d.flags = (int)p.flags;
   d.pow2 = (is_pow2(p.nx) && p.nx >= 8) ? 1 : 0;
+  d.nb = (p.nx + REF_BLOCK - 1) / REF_BLOCK;
   const double cs2 = p.cs2, dt = p.dt;
   // relaxation rates, LBM.cu:336-341
   const double omega_plus = 1.0 / (p.nu / cs2 / dt + 1.0 / 2.0) / dt;
@@ -220,6 +240,7 @@ StepBuffers make_buffers(ekpnp_ctx* c) {
   b.red_first_tile = c->d_first_tile;
   b.n_channels = c->nchan;
   b.refs = c->d_refs;
+  b.blocks = c->d_blocks;
   b.row_tiles = c->d_row_tiles;
   b.n_row_tiles = (int)c->row_tiles.size();
   b.cprime = c->d_cprime;
@@ -247,7 +268,8 @@ void release(ekpnp_ctx* c) {
   void* ptrs[] = {c->lat[0], c->lat[1], c->d_desc, c->d_lbm_tiles, c->d_fft_tiles,
                   c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
                   c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red,
-                  c->d_refs, c->d_refs_new, c->d_row_tiles, c->d_cprime, c->d_lbm_split};
+                  c->d_refs, c->d_refs_new, c->d_row_tiles, c->d_cprime, c->d_lbm_split,
+                  c->d_blocks, c->d_blocks_new};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -490,17 +512,24 @@ hipError_t sync_desc(ekpnp_ctx* c) {
 // lattice index holding channel ch's current state
 int chan_lattice(const ekpnp_ctx* c, int ch) { return c->cur ^ c->desc[ch].swap; }
 
-// [9][ny]: w_d (X(y) + c_dx J(y) / (cs^2 CFL)) of one species
+// [9][ny][nx]: w_d (X(x, y) + c_dx J(y) / (cs^2 CFL)) of one species
 std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err) {
   const ChanDesc& d = c->desc[channel];
-  std::vector<double> rec(kRefStride * (size_t)d.ny), out(9 * (size_t)d.ny);
+  const size_t nn = (size_t)d.nx * d.ny;
+  std::vector<double> rec(kRefStride * (size_t)d.ny), out(9 * nn);
+  std::vector<float> kn(BLK_STRIDE * (size_t)d.nb * d.ny);
   *err = hipMemcpy(rec.data(), c->d_refs + d.ref_off, rec.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (*err == hipSuccess)
+    *err = hipMemcpy(kn.data(), c->d_blocks + d.blk_off, kn.size() * sizeof(float), hipMemcpyDeviceToHost);
   const double tc = d.inv_cs2 * d.inv_cfl;
-  for (int dd = 0; dd < 9; ++dd)
-    for (int y = 0; y < d.ny; ++y) {
-      const double X = rec[kRefStride * (size_t)y + species], J = rec[kRefStride * (size_t)y + 3 + species];
-      out[dd * (size_t)d.ny + y] = W[dd] * (X + CXH[dd] * tc * J);
+  for (int y = 0; y < d.ny; ++y) {
+    const double J = rec[kRefStride * (size_t)y + 3 + species];
+    const float* Br = &kn[BLK_STRIDE * (size_t)d.nb * y];
+    for (int x = 0; x < d.nx; ++x) {
+      const double X = Br[BLK_STRIDE * (x / REF_BLOCK) + species];  // the block's reference
+      for (int dd = 0; dd < 9; ++dd) out[dd * nn + (size_t)y * d.nx + x] = W[dd] * (X + CXH[dd] * tc * J);
     }
+  }
   return out;
 }
 
@@ -516,6 +545,19 @@ hipError_t upload_psi_walls(ekpnp_ctx* c, int ch) {
                      hipMemcpyHostToDevice, c->stream);
   if (e != hipSuccess) return e;
   return hipStreamSynchronize(c->stream);
+}
+
+// Step count at which FP32 storage next re-centres its references: 8, 16,
+// ..., EKPNP_REBASE_INTERVAL steps after init_equilibrium (the fields change
+// fastest while a cold start's double layer and flow form, and the
+// references must follow them to keep the stored deviations small), then
+// every EKPNP_REBASE_INTERVAL steps.  A function of the step counts only, so
+// a run re-centres at the same steps however it is cut into calls.
+long long next_rebase(const ekpnp_ctx* c) {
+  const long long last = c->rebase_step - c->origin_step;
+  const long long next = last < EKPNP_REBASE_INTERVAL ? std::max<long long>(8, 2 * last)
+                                                      : last + EKPNP_REBASE_INTERVAL;
+  return c->origin_step + next;
 }
 
 }  // namespace
@@ -601,6 +643,19 @@ void ekpnp_default_options(ekpnp_options* o) {
   o->device = 0;
   o->use_graph = 1;
   o->compute_fp64 = 1;
+  o->reject_unresolved = 0;
+}
+
+int ekpnp_check_params(const ekpnp_channel_params* channels, int n_channels, uint32_t* flags_out) {
+  if (!channels || n_channels <= 0) return EKPNP_ERR_INVALID;
+  int flagged = 0;
+  for (int i = 0; i < n_channels; ++i) {
+    double kdy = 0.0, drift = 0.0;
+    const uint32_t f = check_flags(channels[i], &kdy, &drift);
+    if (flags_out) flags_out[i] = f;
+    flagged += f != 0;
+  }
+  return flagged;
 }
 
 const char* ekpnp_create_error(void) { return g_create_error.c_str(); }
@@ -629,6 +684,19 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
       g_create_error = m;
       return (m.find("above") != std::string::npos) ? EKPNP_ERR_UNSUPPORTED : EKPNP_ERR_INVALID;
     }
+    double kdy = 0.0, drift = 0.0;
+    const uint32_t f = check_flags(channels[i], &kdy, &drift);
+    if (o.reject_unresolved && f) {
+      char buf[256];
+      snprintf(buf, sizeof buf, "channel %d: %s%s (kappa dy = %.3g, limit 0.4; wall drift = %.3g, limit 0.05)", i,
+               (f & EKPNP_CHECK_DEBYE_UNRESOLVED) ? "Debye length unresolved" : "",
+               (f & EKPNP_CHECK_WALL_DRIFT) ? ((f & EKPNP_CHECK_DEBYE_UNRESOLVED) ? ", wall drift too fast"
+                                                                                   : "wall drift too fast")
+                                            : "",
+               kdy, drift);
+      g_create_error = buf;
+      return EKPNP_ERR_UNSTABLE;
+    }
   }
   hipError_t he = hipSetDevice(o.device);
   if (he != hipSuccess) {
@@ -644,7 +712,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   c->ctiles.resize(n_channels);
 
   // layout
-  long long pop = 0, fld = 0, spec = 0, refs = 0, cprime = 0;
+  long long pop = 0, fld = 0, spec = 0, refs = 0, cprime = 0, blocks = 0;
   int wall = 0;
   // every per-channel block starts on a 64-element boundary
   auto pad64 = [](long long v) { return (v + 63) / 64 * 64; };
@@ -656,6 +724,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     d.fld_off = fld;
     d.spec_off = spec;
     d.ref_off = refs;
+    d.blk_off = blocks;
     d.cp_off = cprime;
     d.wall_off = wall;
     d.tw_off = wall;
@@ -663,6 +732,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     fld += pad64(nn);
     spec += pad64((long long)d.ny * d.nk_stride);
     refs += pad64((long long)kRefStride * d.ny);
+    blocks += pad64((long long)BLK_STRIDE * d.nb * d.ny);
     cprime += pad64((long long)kMaxTriChunks * d.nk);
     wall += (int)pad64(d.nx);
     c->desc[i] = d;
@@ -671,6 +741,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   c->total_nodes = fld;
   c->total_spec = spec;
   c->total_refs = refs;
+  c->total_blocks = blocks;
   c->total_cprime = cprime;
   c->total_nx = wall;
   for (int i = 0; i < n_channels; ++i)
@@ -758,6 +829,8 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = dalloc(c, &c->d_ux_prev, c->total_nodes)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_refs, c->total_refs)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_refs_new, c->total_refs)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_blocks, c->total_blocks)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_blocks_new, c->total_blocks)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_row_tiles, c->row_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_partials, 8 * c->lbm_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_red, 8 * (size_t)n_channels)) != hipSuccess) return e;
@@ -783,6 +856,8 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = hipMemcpy(c->d_row_tiles, c->row_tiles.data(), c->row_tiles.size() * sizeof(int2), hipMemcpyHostToDevice)) != hipSuccess) return e;
     if ((e = hipMemset(c->d_refs, 0, c->total_refs * sizeof(double))) != hipSuccess) return e;
     if ((e = hipMemset(c->d_refs_new, 0, c->total_refs * sizeof(double))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_blocks, 0, c->total_blocks * sizeof(float))) != hipSuccess) return e;
+    if ((e = hipMemset(c->d_blocks_new, 0, c->total_blocks * sizeof(float))) != hipSuccess) return e;
     const size_t npop = (size_t)c->total_pops;
     if ((e = hipMemset(c->lat[0], 0, npop * c->elem)) != hipSuccess) return e;
     if ((e = hipMemset(c->lat[1], 0, npop * c->elem)) != hipSuccess) return e;
@@ -962,13 +1037,15 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
   }
   if (reset) HIP_OR_FAIL(ctx, sync_desc(ctx));
   // FP32 storage: populations are stored as deviations from w_i times the
-  // row means of the fields they start from
-  if (ctx->opt.precision == EKPNP_POP_FP32) HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs, ctx->stream));
+  // references (block means along x, row-mean fluxes) of the fields they start from
+  if (ctx->opt.precision == EKPNP_POP_FP32)
+    HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs, ctx->d_blocks, ctx->stream));
   HIP_OR_FAIL(ctx, launch_init_equilibrium(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   ctx->state_q = true;
   ctx->has_pops = true;
   ctx->rebase_step = ctx->steps;
+  ctx->origin_step = ctx->steps;
   return EKPNP_OK;
 }
 
@@ -1033,10 +1110,13 @@ int ekpnp_rebase(ekpnp_ctx* ctx) {
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   // the macroscopic planes hold the moments of the last step (or the fields
   // the populations were initialised from)
-  HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs_new, ctx->stream));
+  // (frozen channels of run_to_residual keep their references and state)
+  HIP_OR_FAIL(ctx, launch_row_means(ctx->sb, ctx->d_refs_new, ctx->d_blocks_new, ctx->stream));
   HIP_OR_FAIL(ctx, launch_rebase(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1],
-                                 ctx->d_refs_new, ctx->stream));
+                                 ctx->d_refs_new, ctx->d_blocks_new, ctx->stream));
   HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_refs, ctx->d_refs_new, ctx->total_refs * sizeof(double),
+                                  hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_blocks, ctx->d_blocks_new, ctx->total_blocks * sizeof(float),
                                   hipMemcpyDeviceToDevice, ctx->stream));
   return EKPNP_OK;
 }
@@ -1058,11 +1138,11 @@ int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
     if (fp32) {
       // overdue (steps taken by other entry points): re-centre first; the
       // change of representation is exact whatever the references
-      if (ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL) {
+      if (ctx->steps >= next_rebase(ctx)) {
         int rc = ekpnp_rebase(ctx);
         if (rc) return rc;
       }
-      const long long until = EKPNP_REBASE_INTERVAL - (ctx->steps - ctx->rebase_step);
+      const long long until = next_rebase(ctx) - ctx->steps;
       if (until < n) n = (int)until;
     }
     const int seg = n;
@@ -1084,7 +1164,7 @@ int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
     }
     ctx->steps += seg;
     left -= seg;
-    if (fp32 && ctx->steps - ctx->rebase_step >= EKPNP_REBASE_INTERVAL) {
+    if (fp32 && ctx->steps >= next_rebase(ctx)) {
       int rc = ekpnp_rebase(ctx);
       if (rc) return rc;
     }
@@ -1166,8 +1246,7 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
     HIP_OR_FAIL(ctx, hipMemcpy2D(tmp.data(), nn * 4, base, (size_t)d.plane * 4, nn * 4, 9, hipMemcpyDeviceToHost));
     for (size_t i = 0; i < 9 * nn; ++i) host_out[i] = (double)tmp[i];
   }
-  for (int dd = 0; dd < 9; ++dd)
-    for (size_t i = 0; i < nn; ++i) host_out[dd * nn + i] += ref[dd * d.ny + i / d.nx];
+  for (size_t i = 0; i < 9 * nn; ++i) host_out[i] += ref[i];
   return EKPNP_OK;
 }
 
@@ -1193,25 +1272,21 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double
   if (ctx->elem == 8) {
     std::vector<double> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - ref[dd * d.ny + i / d.nx];
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = host_in[dd * nn + i] - ref[dd * nn + i];
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 8, tmp.data(), nn * 8, nn * 8, 9, hipMemcpyHostToDevice));
   } else {
     std::vector<float> tmp(9 * nn);
     for (int dd = 0; dd < 9; ++dd)
-      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - ref[dd * d.ny + i / d.nx]);
+      for (size_t i = 0; i < nn; ++i) tmp[dd * nn + i] = (float)(host_in[dd * nn + i] - ref[dd * nn + i]);
     HIP_OR_FAIL(ctx, hipMemcpy2D(base, (size_t)d.plane * 4, tmp.data(), nn * 4, nn * 4, 9, hipMemcpyHostToDevice));
   }
   ctx->has_pops = true;
   return EKPNP_OK;
 }
 
-int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
-  if (!ctx || !out) return EKPNP_ERR_INVALID;
-  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
-  HIP_OR_FAIL(ctx, launch_reductions(ctx->sb, 1, ctx->stream));
-  HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_red, 8 * (size_t)ctx->nchan * sizeof(double),
-                                  hipMemcpyDeviceToHost, ctx->stream));
-  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+// residuals from the copied records (DUX2 and UX2 over the interval since
+// the last snapshot), and the new snapshot's step
+static void finish_reductions(ekpnp_ctx* ctx, double* out) {
   const long long interval = ctx->snapshot_step >= 0 ? ctx->steps - ctx->snapshot_step : 0;
   for (int ch = 0; ch < ctx->nchan; ++ch) {
     double* r = out + 8 * ch;
@@ -1223,6 +1298,16 @@ int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
     r[EKPNP_RED_RESIDUAL] = res;
   }
   ctx->snapshot_step = ctx->steps;
+}
+
+int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
+  if (!ctx || !out) return EKPNP_ERR_INVALID;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, launch_reductions(ctx->sb, 1, ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_red, 8 * (size_t)ctx->nchan * sizeof(double),
+                                  hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  finish_reductions(ctx, out);
   return EKPNP_OK;
 }
 
@@ -1387,15 +1472,16 @@ int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {
 
 // ---------------------------------------------------------------------------
 // Binary checkpoint.  Layout (native byte order):
-//   char[8] "EKPNPCK1"; int32 nchan, precision, state_q, has_pops;
-//   int64 steps, snapshot_step, rebase_step;
+//   char[8] "EKPNPCK2"; int32 nchan, precision, state_q, has_pops;
+//   int64 steps, snapshot_step, rebase_step, origin_step;
 //   per channel: int32 nx, ny;
 //   per channel: populations [27][ny][nx] in the storage type, references
-//   [ny][8], psi, rho, ux, uy, c+, c- [ny][nx], wall planes [6][nx],
+//   [ny][8], block references [ny][nb][4] (FP32, nb = ceil(nx/64)), psi, rho, ux,
+//   uy, c+, c- [ny][nx], wall planes [6][nx],
 //   residual snapshot [ny][nx] (FP64 unless stated).
 // ---------------------------------------------------------------------------
 namespace {
-const char kCkptMagic[8] = {'E', 'K', 'P', 'N', 'P', 'C', 'K', '1'};
+const char kCkptMagic[8] = {'E', 'K', 'P', 'N', 'P', 'C', 'K', '2'};
 
 struct FileCloser {
   FILE* f;
@@ -1416,7 +1502,7 @@ int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
   bool ok = true;
   auto put = [&](const void* p, size_t n) { ok = ok && std::fwrite(p, 1, n, fp) == n; };
   const int32_t hdr[4] = {ctx->nchan, ctx->opt.precision, ctx->state_q ? 1 : 0, ctx->has_pops ? 1 : 0};
-  const int64_t cnt[3] = {ctx->steps, ctx->snapshot_step, ctx->rebase_step};
+  const int64_t cnt[4] = {ctx->steps, ctx->snapshot_step, ctx->rebase_step, ctx->origin_step};
   put(kCkptMagic, 8);
   put(hdr, sizeof hdr);
   put(cnt, sizeof cnt);
@@ -1436,6 +1522,10 @@ int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
     HIP_OR_FAIL(ctx, hipMemcpy(refs.data(), ctx->d_refs + d.ref_off, refs.size() * sizeof(double),
                                hipMemcpyDeviceToHost));
     put(refs.data(), refs.size() * sizeof(double));
+    std::vector<float> kn(BLK_STRIDE * (size_t)d.nb * d.ny);
+    HIP_OR_FAIL(ctx, hipMemcpy(kn.data(), ctx->d_blocks + d.blk_off, kn.size() * sizeof(float),
+                               hipMemcpyDeviceToHost));
+    put(kn.data(), kn.size() * sizeof(float));
     double* fb = reinterpret_cast<double*>(buf.data());
     HIP_OR_FAIL(ctx, hipMemcpy(fb, ctx->d_psi + d.fld_off, nn * sizeof(double), hipMemcpyDeviceToHost));
     for (int k = 0; k < M_COUNT; ++k)
@@ -1460,7 +1550,7 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
   auto get = [&](void* p, size_t n) { return std::fread(p, 1, n, fp) == n; };
   char magic[8];
   int32_t hdr[4];
-  int64_t cnt[3];
+  int64_t cnt[4];
   if (!get(magic, 8) || std::memcmp(magic, kCkptMagic, 8) != 0 || !get(hdr, sizeof hdr) || !get(cnt, sizeof cnt))
     return fail(ctx, EKPNP_ERR_IO, "not a checkpoint file");
   if (hdr[0] != ctx->nchan || hdr[1] != ctx->opt.precision)
@@ -1488,6 +1578,10 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
     if (!get(refs.data(), refs.size() * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
     HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_refs + d.ref_off, refs.data(), refs.size() * sizeof(double),
                                hipMemcpyHostToDevice));
+    std::vector<float> kn(BLK_STRIDE * (size_t)d.nb * d.ny);
+    if (!get(kn.data(), kn.size() * sizeof(float))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
+    HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_blocks + d.blk_off, kn.data(), kn.size() * sizeof(float),
+                               hipMemcpyHostToDevice));
     double* fb = reinterpret_cast<double*>(buf.data());
     if (!get(fb, (M_COUNT + 1) * nn * sizeof(double))) return fail(ctx, EKPNP_ERR_IO, "truncated checkpoint");
     HIP_OR_FAIL(ctx, hipMemcpy(ctx->d_psi + d.fld_off, fb, nn * sizeof(double), hipMemcpyHostToDevice));
@@ -1509,7 +1603,46 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
   ctx->steps = cnt[0];
   ctx->snapshot_step = cnt[1];
   ctx->rebase_step = cnt[2];
+  ctx->origin_step = cnt[3];
   return EKPNP_OK;
+}
+
+int ekpnp_copy_bandwidth(int device, size_t bytes, int reps, int grid, double* gbps_out, double* ms_out) {
+  if (!gbps_out || reps < 1 || grid < 1 || bytes < 16 || bytes % 16 != 0) return EKPNP_ERR_INVALID;
+  if (hipSetDevice(device) != hipSuccess) return EKPNP_ERR_HIP;
+  void* a = nullptr;
+  void* b = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int rc = EKPNP_OK;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) rc = EKPNP_ERR_NOMEM;
+  if (rc == EKPNP_OK && (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess ||
+                         hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess ||
+                         hipMemsetAsync(a, 0, bytes, st) != hipSuccess))
+    rc = EKPNP_ERR_HIP;
+  const long long n = (long long)(bytes / 16);
+  float ms = 0.0f;
+  if (rc == EKPNP_OK) {
+    // one untimed copy, then reps timed ones
+    hipError_t e = launch_copy_nt(a, b, n, grid, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_copy_nt(a, b, n, grid, st);
+    if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+    if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+    if (e != hipSuccess) rc = EKPNP_ERR_HIP;
+  }
+  if (rc == EKPNP_OK) {
+    const double per = (double)ms / reps;
+    *gbps_out = 2.0 * (double)bytes / (per * 1e-3) / 1e9;
+    if (ms_out) *ms_out = per;
+  }
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (st) (void)hipStreamDestroy(st);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
 }
 
 int ekpnp_num_channels(const ekpnp_ctx* ctx) { return ctx ? ctx->nchan : 0; }

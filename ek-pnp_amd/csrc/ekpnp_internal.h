@@ -32,18 +32,26 @@ constexpr int MAX_NX = 8192;      // LDS-resident x-rows (64 KiB of complex FP64
 constexpr int MAX_NX_DFT = 2048;  // non-power-of-two rows use a direct DFT
 constexpr int TRI_LMAX = 16;              // rows per tridiagonal chunk (one lane)
 constexpr int MAX_INTERIOR_ROWS = 2048;  // 128 chunks x 16 rows per tridiagonal column
+// FP32 storage references along x: one record of BLK_STRIDE floats (rho,
+// c+, c-, unused) per block of REF_BLOCK consecutive nodes of a row (one
+// wave of the row kernel); a reference need not be exact, only the same
+// wherever it is used, and FP32 values halve the scalar registers a wave
+// holds them in
+constexpr int REF_BLOCK = 64;
+constexpr int BLK_STRIDE = 4;
 
 struct ChanDesc {
   int nx, ny, nk, flags;        // nk = nx/2 + 1 spectral columns
   int nk_stride;                // nk rounded up to a multiple of 8 (spectrum blocks)
   int active;                   // 0: the step kernels skip this channel (frozen)
   int swap;                     // 1: this channel's state is in the step's dst lattice
-  int pad0;
+  int nb;                       // reference blocks per row: ceil(nx / REF_BLOCK)
   long long pop_off;            // first element of this channel's 27 planes
   long long plane;              // population plane stride (nx*ny rounded up to 64)
   long long fld_off;            // first element of this channel's [ny][nx] fields
   long long spec_off;           // first double2 of this channel's spectrum blocks
   long long ref_off;            // first double of this channel's [ny][8] storage references
+  long long blk_off;            // first float of this channel's [ny][nb][BLK_STRIDE] block records
   long long cp_off;             // first double of this channel's [tri_C][nk] chunk-entry Thomas coefficients
   int wall_off;                 // offset into each wall plane
   int tw_off;                   // offset of this channel's twiddles e^{-2 pi i j/nx}
@@ -105,7 +113,8 @@ struct StepBuffers {
   double* red_out;              // reductions [n_channels][8]
   const int* red_first_tile;    // first LBM tile of each channel (n_channels + 1)
   int n_channels;
-  double* refs;                 // storage references, [ny][8] per channel (rho, c+, c-, their x-fluxes, 2 unused)
+  double* refs;                 // storage references, [ny][8] per channel (row means of rho, c+, c-, their x-fluxes, 2 unused)
+  float* blocks;                // storage references along x, [ny][nb][BLK_STRIDE] per channel
   const int2* row_tiles;        // (channel, row) per row-means work-group
   int n_row_tiles;
   double* cprime;               // Thomas coefficient entering chunk j of mode k, [tri_C][nk] per channel
@@ -137,12 +146,14 @@ hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s);
 // PB-Picard start: Boltzmann concentrations from psi, then under-relaxation
 // of psi against psi_old (psi_old is updated to the relaxed value)
 hipError_t launch_pb_fields(const StepBuffers& b, hipStream_t s);
-// row means over x of rho, c+, c- and rho u_x, c+ u_x, c- u_x into out ([ny][8] per channel)
-hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s);
-// re-express every stored population against new_refs (state in cur, or in
-// other for channels whose swap flag is set)
+// row means over x of rho, c+, c- and rho u_x, c+ u_x, c- u_x into out ([ny][8] per channel), and the
+// block records (means of rho, c+, c- over each block of REF_BLOCK nodes) into blocks_out; channels that
+// are not active keep the references in force (b.refs, b.blocks)
+hipError_t launch_row_means(const StepBuffers& b, double* out, float* blocks_out, hipStream_t s);
+// re-express every stored population of the active channels against new_refs
+// / new_blocks (state in cur, or in other for channels whose swap flag is set)
 hipError_t launch_rebase(int precision, const StepBuffers& b, void* cur, void* other, const double* new_refs,
-                         hipStream_t s);
+                         const float* new_blocks, hipStream_t s);
 hipError_t launch_pb_relax(const StepBuffers& b, double* psi_old, hipStream_t s);
 // one red-black SOR half-sweep of channel chan's 5-point Poisson system
 // (color 0: x + y even); partials[wg] = largest |change| per work-group,
@@ -152,5 +163,7 @@ hipError_t launch_sor_sweep(const StepBuffers& b, int chan, int nx, int ny, int 
 hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s);
 // psi wall rows of channel chan from the wall potentials
 hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStream_t s);
+// dst = src, n_float4 16-byte values, non-temporal, grid work-groups of 256 lanes
+hipError_t launch_copy_nt(const void* src, void* dst, long long n_float4, int grid, hipStream_t s);
 
 }  // namespace ekpnp

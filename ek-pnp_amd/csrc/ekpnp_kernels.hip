@@ -105,59 +105,111 @@ __device__ __forceinline__ void efield_at(const double* __restrict__ ps, int nx,
   Ey = (ps[(long long)ya * nx + x] - ps[(long long)yb * nx + x]) * inv2dy;
 }
 
-// Storage reference of a row (FP32 storage; all zero for FP64 storage).  R
-// points at the channel's table of one 64-byte record per row:
-// X_s(y) = (rho, c+, c-) and J_s(y) = the x-fluxes (rho u_x, c+ u_x, c- u_x),
-// row means, then two unused slots.  A stored population is
-//   stored_i = f_i - w_i X_s(y) - w_i c_ix tc J_s(y),   tc = 1/(cs^2 CFL),
-// i.e. the departure from the first-order equilibrium of the row means.  The
-// collision works on the rest deviation q_i = f_i - w_i X_s(y).
-struct RowRefs {
-  double X[3], J[3];
-};
+// Storage references (FP32 storage; FP64 storage keeps absolute populations
+// and uses none).  Per channel two tables:
+//   row records R[y] (REF_STRIDE doubles): the row means of X = (rho, c+, c-)
+//     and of their x-fluxes J = (rho u_x, c+ u_x, c- u_x), two unused slots;
+//   block records B[y][k] (BLK_STRIDE floats), k < nb = ceil(nx/64): the
+//     means of X over columns 64 k .. 64 k + 63 of the row, rounded to FP32.
+// The reference of node (x, y) is X(x, y) = B[y][x/64], and a stored
+// population is
+//   stored_i = f_i - w_i X(x, y) - w_i c_ix tc J(y),   tc = 1/(cs^2 CFL),
+// the departure from the first-order equilibrium of the references.  The
+// collision works on the rest deviation q_i = f_i - w_i X(x, y).  Following
+// the double layer along x in 64-node blocks (wall potentials that vary
+// with x) keeps the stored ion deviations, and their FP32 rounding, small,
+// where a row mean alone leaves deviations of the order of the
+// concentration itself.  A block is one wave of the row kernel, so its
+// references are wave-uniform.
 constexpr int REF_ROWS = 6;     // values per row record
 constexpr int REF_STRIDE = 8;   // doubles per row record
-__device__ __forceinline__ RowRefs load_refs(const double* __restrict__ R, int y) {
-  const double2* p = reinterpret_cast<const double2*>(R + (long long)REF_STRIDE * y);
-  const double2 a = p[0], b = p[1], c = p[2];
-  RowRefs r;
-  r.X[0] = a.x;
-  r.X[1] = a.y;
-  r.X[2] = b.x;
-  r.J[0] = b.y;
-  r.J[1] = c.x;
-  r.J[2] = c.y;
-  return r;
-}
-// the row records a node of row y reads: rows y-1, y, y+1 (periodic in y)
-struct Refs3 {
-  RowRefs m, o, p;
+static_assert(REF_BLOCK == 64, "reference blocks are found by shifts");
+
+// x-fluxes J of a row record
+struct RowJ {
+  double J[3];
 };
-__device__ __forceinline__ Refs3 load_refs3(const double* __restrict__ R, int ny, int y) {
+__device__ __forceinline__ RowJ load_rowj(const double* __restrict__ R, int y) {
+  const double* r = R + (long long)REF_STRIDE * y;
+  return RowJ{{r[3], r[4], r[5]}};
+}
+
+// X_s at column x of the row whose block records start at Br
+__device__ __forceinline__ double block_x(const float* __restrict__ Br, int x, int s) {
+  return (double)Br[BLK_STRIDE * (x >> 6) + s];
+}
+
+// The references of one node, generic path: block rows of y-1, y, y+1
+// (periodic in y), the node's columns x-1, x, x+1 and the three row fluxes.
+struct NodeRefs {
+  const float* Bm;
+  const float* Bo;
+  const float* Bp;
+  int x, xm, xp;
+  RowJ jm, jo, jp;
+  __device__ double own(int s) const { return block_x(Bo, x, s); }
+  // X(src) - X(node) for the 9 gather sources of D2Q9 (d[0] = 0)
+  __device__ void diffs(int s, double* d) const {
+    const double o = own(s);
+    d[0] = 0.0;
+    d[1] = block_x(Bo, xm, s) - o;
+    d[2] = block_x(Bm, x, s) - o;
+    d[3] = block_x(Bo, xp, s) - o;
+    d[4] = block_x(Bp, x, s) - o;
+    d[5] = block_x(Bm, xm, s) - o;
+    d[6] = block_x(Bm, xp, s) - o;
+    d[7] = block_x(Bp, xp, s) - o;
+    d[8] = block_x(Bp, xm, s) - o;
+  }
+};
+__device__ __forceinline__ NodeRefs node_refs(const StepBuffers& b, const ChanDesc* __restrict__ cd, int x, int y,
+                                              int xm, int xp) {
+  const int ny = cd->ny;
   const int ym = (y == 0) ? ny - 1 : y - 1;
   const int yp = (y == ny - 1) ? 0 : y + 1;
-  return Refs3{load_refs(R, ym), load_refs(R, y), load_refs(R, yp)};
+  const float* B = b.blocks + cd->blk_off;
+  const double* R = b.refs + cd->ref_off;
+  const long long bs = (long long)BLK_STRIDE * cd->nb;
+  return NodeRefs{B + ym * bs, B + y * bs, B + yp * bs, x, xm, xp,
+                  load_rowj(R, ym), load_rowj(R, y), load_rowj(R, yp)};
 }
 
 // direction x-components c_ix of D2Q9 (0, E, N, W, S, NE, NW, SW, SE)
 __device__ constexpr int CX[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 
-// rest deviation of node (x, y) from stored values at their own rows
+// The gathered species' stored values v (FP32 storage) plus the reference
+// terms of their sources: q_i = v_i + w_i (X(src) - X(node)) + w_i c_ix tc J(src row)
+// with dX = X(src) - X(node) and the row fluxes (times tc) j0 (row y), jm
+// (y-1), jp (y+1).
+template <typename C>
+__device__ __forceinline__ void add_src_refs(C* q, const double* dX, C j0, C jm, C jp) {
+  q[1] += C(WS) * (C(dX[1]) + j0);
+  q[3] += C(WS) * (C(dX[3]) - j0);
+  q[2] += C(WS) * C(dX[2]);
+  q[4] += C(WS) * C(dX[4]);
+  q[5] += C(WD) * (C(dX[5]) + jm);
+  q[6] += C(WD) * (C(dX[6]) - jm);
+  q[7] += C(WD) * (C(dX[7]) - jp);
+  q[8] += C(WD) * (C(dX[8]) + jp);
+}
+
+// rest deviation of node (x, y) from stored values at the node itself
 // (FROM_Q) or gathered from the upstream neighbours (pull streaming):
-//   q_i = stored_i(src) + w_i (X(src) - X(y)) + w_i c_ix tc J(src).
-// 27 values: f 0..8, h 9..17, g 18..26.
-template <typename S, typename C, bool FROM_Q>
+//   q_i = stored_i(src) + w_i (X(src) - X(node)) + w_i c_ix tc J(src).
+// 27 values: f 0..8, h 9..17, g 18..26.  REF false (FP64 storage): no
+// reference terms.
+template <typename S, typename C, bool FROM_Q, bool REF>
 __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plane, int nx, int ny,
-                                         int x, int y, int xm, int xp, const Refs3& rf, double tc, C* q) {
+                                         int x, int y, int xm, int xp, const NodeRefs& rf, double tc, C* q) {
   const long long r0 = (long long)y * nx;
   if (FROM_Q) {
 #pragma unroll
     for (int s = 0; s < 3; ++s) {
-      const C j = C(rf.o.J[s] * tc);
+      const C j = REF ? C(rf.jo.J[s] * tc) : C(0);
 #pragma unroll
       for (int d = 0; d < 9; ++d) {
         q[9 * s + d] = (C)ld_pop(P + (9 * s + d) * plane + r0 + x);
-        if (CX[d] != 0) q[9 * s + d] += C(CX[d] * (d < 5 ? WS : WD)) * j;
+        if (REF && CX[d] != 0) q[9 * s + d] += C(CX[d] * (d < 5 ? WS : WD)) * j;
       }
     }
     return;
@@ -168,9 +220,6 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
     const S* B = P + (long long)(9 * s) * plane;
-    const double own = rf.o.X[s];
-    const C dm = C(rf.m.X[s] - own), dp = C(rf.p.X[s] - own);
-    const C j0 = C(rf.o.J[s] * tc), jm = C(rf.m.J[s] * tc), jp = C(rf.p.J[s] * tc);
     q[9 * s + 0] = (C)ld_pop(B + 0 * plane + r0 + x);
     q[9 * s + 1] = (C)ld_pop(B + 1 * plane + r0 + xm);
     q[9 * s + 2] = (C)ld_pop(B + 2 * plane + rm + x);
@@ -180,14 +229,11 @@ __device__ __forceinline__ void load_pre(const S* __restrict__ P, long long plan
     q[9 * s + 6] = (C)ld_pop(B + 6 * plane + rm + xp);
     q[9 * s + 7] = (C)ld_pop(B + 7 * plane + rp + xp);
     q[9 * s + 8] = (C)ld_pop(B + 8 * plane + rp + xm);
-    q[9 * s + 1] += C(WS) * j0;
-    q[9 * s + 3] -= C(WS) * j0;
-    q[9 * s + 2] += C(WS) * dm;
-    q[9 * s + 5] += C(WD) * (dm + jm);
-    q[9 * s + 6] += C(WD) * (dm - jm);
-    q[9 * s + 4] += C(WS) * dp;
-    q[9 * s + 7] += C(WD) * (dp - jp);
-    q[9 * s + 8] += C(WD) * (dp + jp);
+    if (REF) {
+      double dX[9];
+      rf.diffs(s, dX);
+      add_src_refs(q + 9 * s, dX, C(rf.jo.J[s] * tc), C(rf.jm.J[s] * tc), C(rf.jp.J[s] * tc));
+    }
   }
 }
 
@@ -291,14 +337,14 @@ struct Moments {
 };
 
 template <typename C>
-__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, const RowRefs& rr,
+__device__ __forceinline__ Moments<C> moments(const C* q, const ChanDesc* __restrict__ cd, const double* Xo,
                                               double Ex, double Ey) {
   Moments<C> m;
   m.dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-  m.rho = C(rr.X[0]) + m.dr;
+  m.rho = C(Xo[0]) + m.dr;
   m.dcp = ((q[9] + q[10]) + (q[11] + q[12])) + ((q[13] + q[14]) + (q[15] + q[16])) + q[17];
   m.dcn = ((q[18] + q[19]) + (q[20] + q[21])) + ((q[22] + q[23]) + (q[24] + q[25])) + q[26];
-  m.rhoe = (m.dcp - m.dcn) + C(rr.X[1] - rr.X[2]);
+  m.rhoe = (m.dcp - m.dcn) + C(Xo[1] - Xo[2]);
   const C jx = (q[1] + q[5] + q[8]) - (q[3] + q[6] + q[7]);
   const C jy = (q[2] + q[5] + q[6]) - (q[4] + q[7] + q[8]);
   m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
@@ -331,46 +377,54 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
   const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
   const double* __restrict__ ps = b.psi + cd->fld_off;
-  const double* __restrict__ R = b.refs + cd->ref_off;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
   const bool bottom = (y == 0), top = (y == ny - 1);
   const double tc = cd->inv_cs2 * cd->inv_cfl;
-  // A work-group inside one row (every tile when nx is a multiple of LBM_WG)
-  // reads its three row records with scalar loads.
-  Refs3 rf;
-  if (tile.y % nx + LBM_WG <= nx) rf = load_refs3(R, ny, tile.y / nx);
-  else rf = load_refs3(R, ny, y);
-  const RowRefs& rr = rf.o;
+  // FP32 storage: deviations from the block references (FP64 storage: none)
+  constexpr bool REF = std::is_same<S, float>::value;
+  NodeRefs rf{};
+  double Xo[3] = {0.0, 0.0, 0.0};
+  if (REF) {
+    rf = node_refs(b, cd, x, y, xm, xp);
+#pragma unroll
+    for (int s = 0; s < 3; ++s) Xo[s] = rf.own(s);
+  }
 
   // y = 0: velocity mirrored from node (x,1)'s pre-collision state, force
   // included (LBM.cu:402-443); evaluated first so its registers are free.
   C mux = 0, muy = 0;
   if (bottom) {
     C m1[27];
-    const Refs3 r1 = load_refs3(R, ny, 1);
-    load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, 1, xm, xp, r1, tc, m1);
+    NodeRefs r1{};
+    double X1[3] = {0.0, 0.0, 0.0};
+    if (REF) {
+      r1 = node_refs(b, cd, x, 1, xm, xp);
+#pragma unroll
+      for (int s = 0; s < 3; ++s) X1[s] = r1.own(s);
+    }
+    load_pre<S, C, FROM_Q, REF>(P, plane, nx, ny, x, 1, xm, xp, r1, tc, m1);
     double Ex1, Ey1;
     efield_at(ps, nx, ny, x, xm, xp, 1, cd->inv2dx, cd->inv2dy, Ex1, Ey1);
-    const Moments<C> mm = moments(m1, cd, r1.o, Ex1, Ey1);
+    const Moments<C> mm = moments(m1, cd, X1, Ex1, Ey1);
     mux = -mm.ux;
     muy = -mm.uy;
   }
 
   C q[27];
-  load_pre<S, C, FROM_Q>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
+  load_pre<S, C, FROM_Q, REF>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
   if (!FROM_Q && (bottom || top)) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.X[1]), C(rr.J[1] * tc), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.X[2]), C(rr.J[2] * tc), q + 18);
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - Xo[1]), REF ? C(rf.jo.J[1] * tc) : C(0), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - Xo[2]), REF ? C(rf.jo.J[2] * tc) : C(0), q + 18);
   }
 
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
-  const Moments<C> m = moments(q, cd, rr, Ex, Ey);
+  const Moments<C> m = moments(q, cd, Xo, Ex, Ey);
   C ux = m.ux, uy = m.uy;
   if (bottom) {
     ux = mux;
@@ -385,22 +439,22 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
       sp += (double)q[9 + k];
       sn += (double)q[18 + k];
     }
-    b.rho_e[cd->fld_off + node] = (sp - sn) + (rr.X[1] - rr.X[2]);
+    b.rho_e[cd->fld_off + node] = (sp - sn) + (Xo[1] - Xo[2]);
   }
   if (MACRO) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
     b.macro[M_RHO * st + o] = (double)m.rho;
     b.macro[M_UX * st + o] = (double)ux;
     b.macro[M_UY * st + o] = (double)uy;
-    b.macro[M_CP * st + o] = (double)(C(rr.X[1]) + m.dcp);
-    b.macro[M_CN * st + o] = (double)(C(rr.X[2]) + m.dcn);
+    b.macro[M_CP * st + o] = (double)(C(Xo[1]) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(Xo[2]) + m.dcn);
   }
 
   const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
   // ions: drift velocity u + K E (LBM.cu:479-488)
   {
     const C Eix = C(Ex + cd->drift_app), Eiy = C(Ey);
-    const C cp = C(rr.X[1]) + m.dcp, cn = C(rr.X[2]) + m.dcn;
+    const C cp = C(Xo[1]) + m.dcp, cn = C(Xo[2]) + m.dcn;
     const C kp = C(cd->kp), kn = C(cd->kn);
     const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
     const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
@@ -431,7 +485,7 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
   }
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const C j = C(rr.J[s] * tc);
+    const C j = REF ? C(rf.jo.J[s] * tc) : C(0);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop(Q + (9 * s + d) * plane + node, (S)to_stored(q[9 * s + d], d, j));
   }
@@ -441,42 +495,73 @@ __global__ __launch_bounds__(LBM_WG) LBM_WAVES_ATTR void k_lbm(StepBuffers b, co
 // k_lbm_rows: k_lbm for FP32 storage with FP64 arithmetic, for channels whose
 // rows are whole work-groups (nx a multiple of LBM_WG), after the
 // pre-collision (P') state.  Same operations in the same order as k_lbm, so
-// the same results, with fewer registers: the row index and the three row
-// records are work-group uniform (scalar registers), and the 27 values read
-// stay FP32, each species being widened to its 9 FP64 rest deviations only
-// while its moments or its collision are computed (and stored right after
-// its collision).  Wall rows take k_lbm's wall rules: the y = 0 velocity
-// mirror, the ions' anti-bounce-back of the node's own post-collision values
-// and the fluid's full-way bounce-back.
+// the same results, with fewer registers: the row index, the reference block and
+// the reference records are wave-uniform (scalar loads), and the 27 values
+// read stay FP32, each species being widened to its 9 FP64 rest deviations
+// only while its moments or its collision are computed (and stored right
+// after its collision).  Wall rows take k_lbm's wall rules: the y = 0
+// velocity mirror, the ions' anti-bounce-back of the node's own
+// post-collision values and the fluid's full-way bounce-back.
 // ---------------------------------------------------------------------------
-// The row records a node reads (rows y-1, y, y+1: m, o, p), wave-uniform,
-// held in scalar registers
-struct RefsReg {
-  Refs3 r;
-  __device__ double mX(int s) const { return r.m.X[s]; }
-  __device__ double oX(int s) const { return r.o.X[s]; }
-  __device__ double pX(int s) const { return r.p.X[s]; }
-  __device__ double mJ(int s) const { return r.m.J[s]; }
-  __device__ double oJ(int s) const { return r.o.J[s]; }
-  __device__ double pJ(int s) const { return r.p.J[s]; }
+// The references of a wave's 64 nodes, block k of row y (nx a multiple of
+// 64): the block records k-1, k, k+1 of rows y-1, y, y+1 and those rows'
+// records, all wave-uniform.
+// Wave-uniform reference values are read through the constant address
+// space: the step kernels never write the reference tables, and uniform
+// constant loads become scalar loads into SGPRs (generic pointers would take
+// vector loads, two VGPRs per value).
+typedef const __attribute__((address_space(4))) double cdouble;
+typedef const __attribute__((address_space(4))) float cfloat;
+__device__ __forceinline__ cdouble* as_const(const double* p) { return (cdouble*)p; }
+__device__ __forceinline__ cfloat* as_const(const float* p) { return (cfloat*)p; }
+
+struct WaveRefs {
+  cfloat* B[3];   // block records of rows y-1, y, y+1
+  cdouble* R[3];  // row records of rows y-1, y, y+1
+  int km, k, kp;  // blocks k-1, k, k+1 (periodic)
+  int lane;
+  __device__ double X(int r, int blk, int s) const { return (double)B[r][BLK_STRIDE * blk + s]; }
+  __device__ double own(int s) const { return X(1, k, s); }
+  __device__ double J(int r, int s) const { return R[r][3 + s]; }
+  // X(src) - X(node) for the 9 gather sources (NodeRefs::diffs): the
+  // sources of lane 0 in column x-1 and of lane 63 in column x+1 lie in the
+  // neighbouring blocks, every other source in the wave's own block
+  __device__ void diffs(int s, double* d) const {
+    const double o = own(s);
+    const bool l0 = lane == 0, l63 = lane == REF_BLOCK - 1;
+    const double bm = X(0, k, s) - o, bp = X(2, k, s) - o;
+    d[0] = 0.0;
+    d[1] = l0 ? X(1, km, s) - o : 0.0;
+    d[2] = bm;
+    d[3] = l63 ? X(1, kp, s) - o : 0.0;
+    d[4] = bp;
+    d[5] = l0 ? X(0, km, s) - o : bm;
+    d[6] = l63 ? X(0, kp, s) - o : bm;
+    d[7] = l63 ? X(2, kp, s) - o : bp;
+    d[8] = l0 ? X(2, km, s) - o : bp;
+  }
 };
+__device__ __forceinline__ WaveRefs wave_refs(const StepBuffers& b, const ChanDesc* __restrict__ cd, int y, int k,
+                                              int lane) {
+  const int ny = cd->ny, nb = cd->nb;
+  const int ym = (y == 0) ? ny - 1 : y - 1;
+  const int yp = (y == ny - 1) ? 0 : y + 1;
+  const float* B = b.blocks + cd->blk_off;
+  const double* R = b.refs + cd->ref_off;
+  const long long bs = (long long)BLK_STRIDE * nb;
+  return WaveRefs{{as_const(B + ym * bs), as_const(B + y * bs), as_const(B + yp * bs)},
+                  {as_const(R + (long long)REF_STRIDE * ym), as_const(R + (long long)REF_STRIDE * y),
+                   as_const(R + (long long)REF_STRIDE * yp)},
+                  k == 0 ? nb - 1 : k - 1, k, k + 1 == nb ? 0 : k + 1, lane};
+}
 
 // FP64 rest deviations of species sp from its gathered FP32 values (load_pre)
-template <typename RF>
-__device__ __forceinline__ void widen_species(const float* v, const RF& rf, int sp, double tc, double* q) {
-  const double own = rf.oX(sp);
-  const double dm = rf.mX(sp) - own, dp = rf.pX(sp) - own;
-  const double j0 = rf.oJ(sp) * tc, jm = rf.mJ(sp) * tc, jp = rf.pJ(sp) * tc;
+__device__ __forceinline__ void widen_species(const float* v, const WaveRefs& rf, int sp, double tc, double* q) {
 #pragma unroll
   for (int d = 0; d < 9; ++d) q[d] = (double)v[d];
-  q[1] += WS * j0;
-  q[3] -= WS * j0;
-  q[2] += WS * dm;
-  q[5] += WD * (dm + jm);
-  q[6] += WD * (dm - jm);
-  q[4] += WS * dp;
-  q[7] += WD * (dp - jp);
-  q[8] += WD * (dp + jp);
+  double dX[9];
+  rf.diffs(sp, dX);
+  add_src_refs(q, dX, rf.J(1, sp) * tc, rf.J(0, sp) * tc, rf.J(2, sp) * tc);
 }
 
 // wall node, one ion species: anti-bounce-back of its own stored
@@ -497,10 +582,9 @@ __device__ __forceinline__ void abb_species(const float* p, double dcw, double j
 
 // the 9 values of species s at row y (s = 0: gathered; ions: gathered, or on
 // a wall row anti-bounced from the node's own values, dcw = c_w - X_s)
-template <typename RF>
-__device__ __forceinline__ void species_q(const float* v, const RF& rf, int s, double tc, bool wall, double dcw,
-                                          double* q) {
-  if (s > 0 && wall) abb_species(v, dcw, rf.oJ(s) * tc, q);
+__device__ __forceinline__ void species_q(const float* v, const WaveRefs& rf, int s, double tc, bool wall,
+                                          double dcw, double* q) {
+  if (s > 0 && wall) abb_species(v, dcw, rf.J(1, s) * tc, q);
   else widen_species(v, rf, s, tc, q);
 }
 
@@ -521,16 +605,19 @@ __device__ __forceinline__ void gather9(const float* __restrict__ B, long long p
 }
 
 // one node of a row tile (WALL: y = 0 or ny-1); a separate instantiation per
-// case keeps the interior code free of the wall rules' registers
-template <bool MACRO, bool WALL, typename RF>
+// case keeps the interior code free of the wall rules' registers.  macro (the
+// step writes the macroscopic planes) is a run-time, wave-uniform flag: the
+// compiler allocates the interior kernel in 118 VGPRs (4 waves per SIMD) this
+// way, and in 136 (3 waves) when the flag is a template parameter and false.
+template <bool WALL>
 __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDesc* __restrict__ cd, int y, int x,
-                                             const RF& rf, const float* __restrict__ P, float* __restrict__ Q) {
+                                             const WaveRefs& rf, const float* __restrict__ P, float* __restrict__ Q,
+                                             bool macro) {
   using C = double;
   const int nx = cd->nx, ny = cd->ny;
   const long long node = (long long)y * nx + x;
   const long long plane = cd->plane;
   const double* __restrict__ ps = b.psi + cd->fld_off;
-  const double* __restrict__ R = b.refs + cd->ref_off;
   const double tc = cd->inv_cs2 * cd->inv_cfl;
   const int xm = (x == 0) ? nx - 1 : x - 1;
   const int xp = (x == nx - 1) ? 0 : x + 1;
@@ -551,12 +638,16 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
       for (int s = 1; s < 3; ++s) gather9(P + (long long)(9 * s) * plane, plane, rm, r0, rp, x, xm, xp, v + 9 * s);
     }
   }
+  // the node's own references X(x, y)
+  double Xo[3];
+#pragma unroll
+  for (int s = 0; s < 3; ++s) Xo[s] = rf.own(s);
   double dcwp = 0.0, dcwn = 0.0;
   if (wall) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
-    dcwp = (bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws]) - rf.oX(1);
-    dcwn = (bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws]) - rf.oX(2);
+    dcwp = (bottom ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws]) - Xo[1];
+    dcwn = (bottom ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws]) - Xo[2];
   }
   double Ex, Ey;
   efield_at(ps, nx, ny, x, xm, xp, y, cd->inv2dx, cd->inv2dy, Ex, Ey);
@@ -578,7 +669,7 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
   // included (LBM.cu:402-443)
   C mux = 0, muy = 0;
   if (bottom) {
-    const RefsReg r1{load_refs3(R, ny, 1)};
+    const WaveRefs r1 = wave_refs(b, cd, 1, rf.k, rf.lane);
     C q[9], dr, jx, jy, dcp, dcn;
     widen_species(w, r1, 0, tc, q);
     dr = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
@@ -588,8 +679,9 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     dcp = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
     widen_species(w + 18, r1, 2, tc, q);
     dcn = ((q[0] + q[1]) + (q[2] + q[3])) + ((q[4] + q[5]) + (q[6] + q[7])) + q[8];
-    const C rho = C(r1.oX(0)) + dr;
-    const C rhoe = (dcp - dcn) + C(r1.oX(1) - r1.oX(2));
+    const double x0 = r1.own(0), x1 = r1.own(1), x2 = r1.own(2);
+    const C rho = C(x0) + dr;
+    const C rhoe = (dcp - dcn) + C(x1 - x2);
     const C Fx = C(cd->fcoef) * rhoe * C(Ex1 + cd->e_app) + C(cd->exf);
     const C Fy = C(cd->fcoef) * rhoe * C(Ey1);
     const C irho = C(1.0) / rho;
@@ -617,8 +709,8 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
 #pragma unroll
     for (int k = 0; k < 9; ++k) sn += q[k];
   }
-  m.rho = C(rf.oX(0)) + m.dr;
-  m.rhoe = (m.dcp - m.dcn) + C(rf.oX(1) - rf.oX(2));
+  m.rho = C(Xo[0]) + m.dr;
+  m.rhoe = (m.dcp - m.dcn) + C(Xo[1] - Xo[2]);
   m.Fx = C(cd->fcoef) * m.rhoe * C(Ex + cd->e_app) + C(cd->exf);
   m.Fy = C(cd->fcoef) * m.rhoe * C(Ey);
   {
@@ -628,14 +720,14 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     m.uy = (jy * icfl + m.Fy * hdt) * irho;
   }
   const C ux = bottom ? mux : m.ux, uy = bottom ? muy : m.uy;
-  b.rho_e[cd->fld_off + node] = (sp - sn) + (rf.oX(1) - rf.oX(2));
-  if (MACRO) {
+  b.rho_e[cd->fld_off + node] = (sp - sn) + (Xo[1] - Xo[2]);
+  if (macro) {
     const long long o = cd->fld_off + node, st = b.macro_stride;
     b.macro[M_RHO * st + o] = (double)m.rho;
     b.macro[M_UX * st + o] = (double)ux;
     b.macro[M_UY * st + o] = (double)uy;
-    b.macro[M_CP * st + o] = (double)(C(rf.oX(1)) + m.dcp);
-    b.macro[M_CN * st + o] = (double)(C(rf.oX(2)) + m.dcn);
+    b.macro[M_CP * st + o] = (double)(C(Xo[1]) + m.dcp);
+    b.macro[M_CN * st + o] = (double)(C(Xo[2]) + m.dcn);
   }
 
   const C ics = C(cd->inv_cs2), icfl = C(cd->inv_cfl), ticf = ics * icfl;
@@ -646,23 +738,23 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
     const C om_f = C(-0.5) * (ux * ux + uy * uy) * ics;
     C q[9];
     {
-      const C cp = C(rf.oX(1)) + m.dcp;
+      const C cp = C(Xo[1]) + m.dcp;
       const C vpx = ux + kp * Eix, vpy = uy + kp * Eiy;
       const C om_p = C(-0.5) * (vpx * vpx + vpy * vpy) * ics;
       species_q(v + 9, rf, 1, tc, wall, dcwp, q);
       collide_ion(q, m.dcp, cp, vpx, vpy, C(cd->om_pp), C(cd->om_mp), ics, ticf, om_p);
-      const C j = C(rf.oJ(1) * tc);
+      const C j = C(rf.J(1, 1) * tc);
 #pragma unroll
       for (int d = 0; d < 9; ++d) st_pop(Q + (9 + d) * plane + node, (float)to_stored(q[d], d, j));
     }
     {
-      const C cn = C(rf.oX(2)) + m.dcn;
+      const C cn = C(Xo[2]) + m.dcn;
       const C vnx = ux + kn * Eix, vny = uy + kn * Eiy;
       const C om_n = C(-0.5) * (vnx * vnx + vny * vny) * ics;
       species_q(v + 18, rf, 2, tc, wall, dcwn, q);
       collide_ion(q, m.dcn, cn, vnx, vny, C(cd->om_pn), C(cd->om_mn), ics, ticf,
                   (cd->flags & FLAG_G0_FLUID_U) ? om_f : om_n);
-      const C j = C(rf.oJ(2) * tc);
+      const C j = C(rf.J(1, 2) * tc);
 #pragma unroll
       for (int d = 0; d < 9; ++d) st_pop(Q + (18 + d) * plane + node, (float)to_stored(q[d], d, j));
     }
@@ -688,20 +780,21 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
       collide_fluid(q, m.dr, m.rho, ux, uy, m.Fx, m.Fy, C(cd->om_pf), C(cd->om_mf), ics, icfl,
                     C(cd->src_p * cd->dt), C(cd->src_m * cd->dt));
     }
-    const C j = C(rf.oJ(0) * tc);
+    const C j = C(rf.J(1, 0) * tc);
 #pragma unroll
     for (int d = 0; d < 9; ++d) st_pop(Q + d * plane + node, (float)to_stored(q[d], d, j));
   }
 }
 
-// Channels with nx a multiple of 64: the 64 nodes of a wave lie in one row,
-// so the row index and its records are wave-uniform (scalar).  WALL: this
+// Channels with nx a multiple of 64: the 64 nodes of a wave lie in one row
+// and one reference block, so the row index and its references are
+// wave-uniform (scalar).  WALL: this
 // launch updates the waves on wall rows, the other one the waves on interior
 // rows (one kernel holding both paths would need the registers of both); a
 // tile holding both kinds of row is in both launches.
-template <bool MACRO, bool WALL>
+template <bool WALL>
 __device__ __forceinline__ void lbm_rows_tile(const StepBuffers& b, int2 tile, const float* __restrict__ src,
-                                              float* __restrict__ dst) {
+                                              float* __restrict__ dst, bool macro) {
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
   if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
@@ -710,16 +803,18 @@ __device__ __forceinline__ void lbm_rows_tile(const StepBuffers& b, int2 tile, c
   const int y = __builtin_amdgcn_readfirstlane(node0 / nx);
   if ((y == 0 || y == ny - 1) != WALL) return;         // the other launch's wave
   const int x = tile.y + (int)threadIdx.x - y * nx;
+  const int k = __builtin_amdgcn_readfirstlane((node0 - y * nx) >> 6);  // the wave's reference block
   const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
-  const RefsReg rf{load_refs3(b.refs + cd->ref_off, ny, y)};
-  lbm_row_node<MACRO, WALL>(b, cd, y, x, rf, P, Q);
+  const WaveRefs rf = wave_refs(b, cd, y, k, (int)threadIdx.x & 63);
+  lbm_row_node<WALL>(b, cd, y, x, rf, P, Q, macro);
 }
 
-template <bool MACRO, bool WALL>
+template <bool WALL>
 __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
-                                                    const float* __restrict__ src, float* __restrict__ dst) {
-  lbm_rows_tile<MACRO, WALL>(b, tiles[xcd_remap(blockIdx.x, gridDim.x)], src, dst);
+                                                    const float* __restrict__ src, float* __restrict__ dst,
+                                                    int macro) {
+  lbm_rows_tile<WALL>(b, tiles[xcd_remap(blockIdx.x, gridDim.x)], src, dst, macro != 0);
 }
 
 
@@ -750,15 +845,17 @@ __global__ __launch_bounds__(LBM_WG) void k_init_eq(StepBuffers b, S* __restrict
   const double vel[3][2] = {{ux, uy}, {ux + cd->kp * Ex, uy + cd->kp * Ey},
                             {ux + cd->kn * Ex, uy + cd->kn * Ey}};
   const double X[3] = {rho, cp, cn};
-  const RowRefs rr = load_refs(b.refs + cd->ref_off, y);
+  constexpr bool REF = std::is_same<S, float>::value;
+  NodeRefs rf{};
+  if (REF) rf = node_refs(b, cd, x, y, xm, xp);
   S* __restrict__ Q = dst + cd->pop_off;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const double j = rr.J[s] * ticf;
+    const double j = REF ? rf.jo.J[s] * ticf : 0.0;
     const double vx = vel[s][0], vy = vel[s][1];
     const double om = -0.5 * (vx * vx + vy * vy) * ics;
     const double tx = vx * ticf, ty = vy * ticf;
-    const double dX = X[s] - rr.X[s];
+    const double dX = X[s] - (REF ? rf.own(s) : 0.0);
     const double t[9] = {0.0, tx, ty, -tx, -ty, tx + ty, ty - tx, -(tx + ty), tx - ty};
 #pragma unroll
     for (int d = 0; d < 9; ++d) {
@@ -785,24 +882,25 @@ __global__ __launch_bounds__(LBM_WG) void k_to_q(StepBuffers b, const S* __restr
   const long long plane = cd->plane;
   const S* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   const int xm = (x == 0) ? nx - 1 : x - 1, xp = (x == nx - 1) ? 0 : x + 1;
-  const double* __restrict__ R = b.refs + cd->ref_off;
-  const Refs3 rf = load_refs3(R, ny, y);
-  const RowRefs& rr = rf.o;
+  constexpr bool REF = std::is_same<S, float>::value;
+  NodeRefs rf{};
+  if (REF) rf = node_refs(b, cd, x, y, xm, xp);
   const double tc = cd->inv_cs2 * cd->inv_cfl;
   C q[27];
-  load_pre<S, C, false>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
+  load_pre<S, C, false, REF>(P, plane, nx, ny, x, y, xm, xp, rf, tc, q);
   if (y == 0 || y == ny - 1) {
     const double* __restrict__ Wl = b.walls + cd->wall_off + x;
     const long long ws = b.walls_stride;
     const double cwp = (y == 0) ? Wl[W_CP_LO * ws] : Wl[W_CP_HI * ws];
     const double cwn = (y == 0) ? Wl[W_CN_LO * ws] : Wl[W_CN_HI * ws];
-    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - rr.X[1]), C(rr.J[1] * tc), q + 9);
-    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - rr.X[2]), C(rr.J[2] * tc), q + 18);
+    const double X1 = REF ? rf.own(1) : 0.0, X2 = REF ? rf.own(2) : 0.0;
+    abb_own<S, C>(P + 9 * plane, plane, node, C(cwp - X1), REF ? C(rf.jo.J[1] * tc) : C(0), q + 9);
+    abb_own<S, C>(P + 18 * plane, plane, node, C(cwn - X2), REF ? C(rf.jo.J[2] * tc) : C(0), q + 18);
   }
   S* __restrict__ Q = (cd->swap ? const_cast<S*>(src) : dst) + cd->pop_off;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const C j = C(rr.J[s] * tc);
+    const C j = REF ? C(rf.jo.J[s] * tc) : C(0);
 #pragma unroll
     for (int d = 0; d < 9; ++d) Q[(9 * s + d) * plane + node] = (S)to_stored(q[9 * s + d], d, j);
   }
@@ -1831,7 +1929,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag(StepBuffers b) {
 // The tridiagonal launch of a coupled step with the wall rows of the step's
 // collide-stream added: the first nw work-groups (the wall tiles, padded to a
 // multiple of 8 so that the tridiagonal tiles keep their XCD placement) update
-// the row-kernel wall tiles (as k_lbm_rows<MACRO, true>), the rest solve the
+// the row-kernel wall tiles (as k_lbm_rows<true>), the rest solve the
 // tridiagonal tiles.  Wall work-groups go first so that they start with the
 // first round of tridiagonal tiles instead of waiting for a late slot.  Valid because the
 // wall-row update reads only the step's source populations, psi and the
@@ -1850,7 +1948,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag_walls(StepBuffers b, const f
   if (i >= nw)
     tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(i - nw, nt)], sA, sB);
   else if (i < b.n_lbm_wall_tiles)
-    lbm_rows_tile<MACRO, true>(b, b.lbm_wall_tiles[i], src, dst);
+    lbm_rows_tile<true>(b, b.lbm_wall_tiles[i], src, dst, MACRO);
 }
 
 // table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
@@ -1886,7 +1984,9 @@ __global__ __launch_bounds__(LBM_WG) void k_reduce_partial(StepBuffers b, int sn
     const int y = node / nx;
     const long long o = cd->fld_off + node, st = b.macro_stride;
     const double rho = b.macro[M_RHO * st + o], ux = b.macro[M_UX * st + o];
+    const double uy = b.macro[M_UY * st + o];
     const double cp = b.macro[M_CP * st + o], cn = b.macro[M_CN * st + o];
+    const double ps = b.psi[o];
     const double uxp = b.ux_prev[o];
     v[0] = rho;
     v[1] = cp;
@@ -1896,7 +1996,8 @@ __global__ __launch_bounds__(LBM_WG) void k_reduce_partial(StepBuffers b, int sn
       v[4] = (ux - uxp) * (ux - uxp);
       v[5] = ux * ux;
     }
-    v[6] = (isfinite(rho) && isfinite(ux) && isfinite(cp) && isfinite(cn)) ? 0.0 : 1.0;
+    v[6] = (isfinite(rho) && isfinite(ux) && isfinite(uy) && isfinite(cp) && isfinite(cn) && isfinite(ps)) ? 0.0
+                                                                                                          : 1.0;
     if (snapshot) b.ux_prev[o] = ux;
   }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1937,15 +2038,25 @@ __global__ __launch_bounds__(256) void k_reduce_final(StepBuffers b) {
 
 // ---------------------------------------------------------------------------
 
-// Storage references: row means of rho, c+, c- over x (one work-group per
-// channel row), and the exact change of reference of every stored population
-// (delta += w_i (ref_old - ref_new), in FP64, rounded once).
+// Storage references: row means of rho, c+, c- and their x-fluxes (one
+// work-group per channel row) and the row's block records, then the exact
+// change of reference of every stored population (delta += w_i (ref_old -
+// ref_new), in FP64, rounded once).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_row_means(StepBuffers b, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_row_means(StepBuffers b, double* __restrict__ out,
+                                                   float* __restrict__ blocks_out) {
   __shared__ double red[4][REF_ROWS];
   const int2 t = b.row_tiles[blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + t.x;
-  const int nx = cd->nx, y = t.y;
+  const int nx = cd->nx, y = t.y, nb = cd->nb;
+  const long long rec = cd->ref_off + (long long)REF_STRIDE * y;
+  const long long kn0 = cd->blk_off + (long long)BLK_STRIDE * nb * y;
+  if (!cd->active) {
+    // a frozen channel keeps the references its stored state is expressed in
+    for (int i = threadIdx.x; i < REF_STRIDE; i += 256) out[rec + i] = b.refs[rec + i];
+    for (int i = threadIdx.x; i < BLK_STRIDE * nb; i += 256) blocks_out[kn0 + i] = b.blocks[kn0 + i];
+    return;
+  }
   const long long o = cd->fld_off + (long long)y * nx, st = b.macro_stride;
   double v[REF_ROWS] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (int x = threadIdx.x; x < nx; x += 256) {
@@ -1964,34 +2075,62 @@ __global__ __launch_bounds__(256) void k_row_means(StepBuffers b, double* __rest
     const double s = wave_sum(v[k]);
     if (lane == 0) red[wv][k] = s;
   }
+  // blocks: the mean over columns 64 k .. min(64 k + 64, nx) - 1, one wave
+  // per block
+  for (int k = wv; k < nb; k += 4) {
+    const int x = REF_BLOCK * k + lane, cnt = min(REF_BLOCK, nx - REF_BLOCK * k);
+    double br = 0.0, bp = 0.0, bc = 0.0;
+    if (lane < cnt) {
+      br = b.macro[M_RHO * st + o + x];
+      bp = b.macro[M_CP * st + o + x];
+      bc = b.macro[M_CN * st + o + x];
+    }
+    br = wave_sum(br);
+    bp = wave_sum(bp);
+    bc = wave_sum(bc);
+    if (lane == 0) {
+      float* bout = blocks_out + kn0 + (long long)BLK_STRIDE * k;
+      bout[0] = (float)(br / (double)cnt);
+      bout[1] = (float)(bp / (double)cnt);
+      bout[2] = (float)(bc / (double)cnt);
+      bout[3] = 0.0f;
+    }
+  }
   __syncthreads();
-  if (threadIdx.x < REF_ROWS) {
-    const double s = (red[0][threadIdx.x] + red[1][threadIdx.x]) + (red[2][threadIdx.x] + red[3][threadIdx.x]);
+  if (threadIdx.x < REF_STRIDE) {
+    const int i = threadIdx.x;
+    const double s = i < REF_ROWS ? (red[0][i] + red[1][i]) + (red[2][i] + red[3][i]) : 0.0;
     // wall rows store reversed (fluid) or wall-rule (ions) populations whose
     // x-flux is not the row's macroscopic one: no flux reference there
     const bool wall = (y == 0 || y == cd->ny - 1);
-    out[cd->ref_off + (long long)REF_STRIDE * y + threadIdx.x] =
-        (wall && threadIdx.x >= 3) ? 0.0 : s / (double)nx;
+    out[rec + i] = (i >= REF_ROWS || (wall && i >= 3)) ? 0.0 : s / (double)nx;
   }
 }
 
+// Frozen channels (not active) are left alone: their state may sit in either
+// lattice, and k_row_means kept their references.
 template <typename S>
 __global__ __launch_bounds__(LBM_WG) void k_rebase(StepBuffers b, S* __restrict__ cur, S* __restrict__ other,
-                                                   const double* __restrict__ new_refs) {
+                                                   const double* __restrict__ new_refs,
+                                                   const float* __restrict__ new_blocks) {
   const int2 tile = b.lbm_tiles[blockIdx.x];
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
-  const int nx = cd->nx, ny = cd->ny;
+  if (!cd->active) return;
+  const int nx = cd->nx, ny = cd->ny, nb = cd->nb;
   const int node = tile.y + (int)threadIdx.x;
   if (node >= nx * ny) return;
-  const int y = node / nx;
+  const int y = node / nx, x = node - y * nx;
   S* __restrict__ P = (cd->swap ? other : cur) + cd->pop_off;
-  const double* __restrict__ R0 = b.refs + cd->ref_off;
-  const double* __restrict__ R1 = new_refs + cd->ref_off;
+  const double* __restrict__ R0 = b.refs + cd->ref_off + (long long)REF_STRIDE * y;
+  const double* __restrict__ R1 = new_refs + cd->ref_off + (long long)REF_STRIDE * y;
+  const long long krow = cd->blk_off + (long long)BLK_STRIDE * nb * y;
+  const float* __restrict__ K0 = b.blocks + krow;
+  const float* __restrict__ K1 = new_blocks + krow;
   const double tc = cd->inv_cs2 * cd->inv_cfl;
 #pragma unroll
   for (int s = 0; s < 3; ++s) {
-    const double d = R0[REF_STRIDE * y + s] - R1[REF_STRIDE * y + s];
-    const double dj = (R0[REF_STRIDE * y + 3 + s] - R1[REF_STRIDE * y + 3 + s]) * tc;
+    const double d = block_x(K0, x, s) - block_x(K1, x, s);
+    const double dj = (R0[3 + s] - R1[3 + s]) * tc;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       const double w = (k == 0) ? W0 : (k < 5 ? WS : WD);
@@ -2111,6 +2250,27 @@ __global__ __launch_bounds__(256) void k_psi_wall_rows(StepBuffers b, int chan) 
 }
 
 // ---------------------------------------------------------------------------
+// Streaming-copy ceiling: dst = src over n float4 values with non-temporal
+// loads and stores, grid-stride (the fastest plain copy measured on this
+// part: the reference rate for the LBM kernel's HBM traffic and the kernel
+// against which the FETCH_SIZE / WRITE_SIZE counters are calibrated)
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy_nt(const v4f* __restrict__ src, v4f* __restrict__ dst, long long n) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const v4f v = __builtin_nontemporal_load(src + i);
+    __builtin_nontemporal_store(v, dst + i);
+  }
+}
+
+hipError_t launch_copy_nt(const void* src, void* dst, long long n_float4, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_nt, dim3(grid), dim3(256), 0, s, static_cast<const v4f*>(src), static_cast<v4f*>(dst),
+                     n_float4);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename S, typename C>
@@ -2138,16 +2298,12 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, bool walls, const StepBu
   if (rows) {
     const float* fs = reinterpret_cast<const float*>(src);
     float* fd = reinterpret_cast<float*>(dst);
-    if (walls && b.n_lbm_wall_tiles > 0) {
-      const dim3 grid(b.n_lbm_wall_tiles);
-      if (macro) hipLaunchKernelGGL((k_lbm_rows<true, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
-      else hipLaunchKernelGGL((k_lbm_rows<false, true>), grid, block, 0, s, b, b.lbm_wall_tiles, fs, fd);
-    }
-    if (b.n_lbm_row_tiles > 0) {
-      const dim3 grid(b.n_lbm_row_tiles);
-      if (macro) hipLaunchKernelGGL((k_lbm_rows<true, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
-      else hipLaunchKernelGGL((k_lbm_rows<false, false>), grid, block, 0, s, b, b.lbm_row_tiles, fs, fd);
-    }
+    if (walls && b.n_lbm_wall_tiles > 0)
+      hipLaunchKernelGGL((k_lbm_rows<true>), dim3(b.n_lbm_wall_tiles), block, 0, s, b, b.lbm_wall_tiles, fs, fd,
+                         macro ? 1 : 0);
+    if (b.n_lbm_row_tiles > 0)
+      hipLaunchKernelGGL((k_lbm_rows<false>), dim3(b.n_lbm_row_tiles), block, 0, s, b, b.lbm_row_tiles, fs, fd,
+                         macro ? 1 : 0);
   }
   return hipGetLastError();
 }
@@ -2248,20 +2404,20 @@ hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, voi
   return hipGetLastError();
 }
 
-hipError_t launch_row_means(const StepBuffers& b, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_row_means, dim3(b.n_row_tiles), dim3(256), 0, s, b, out);
+hipError_t launch_row_means(const StepBuffers& b, double* out, float* blocks_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_means, dim3(b.n_row_tiles), dim3(256), 0, s, b, out, blocks_out);
   return hipGetLastError();
 }
 
 hipError_t launch_rebase(int precision, const StepBuffers& b, void* cur, void* other, const double* new_refs,
-                         hipStream_t s) {
+                         const float* new_blocks, hipStream_t s) {
   const dim3 grid(b.n_lbm_tiles), block(LBM_WG);
   if (precision == 0)
     hipLaunchKernelGGL((k_rebase<float>), grid, block, 0, s, b, static_cast<float*>(cur),
-                       static_cast<float*>(other), new_refs);
+                       static_cast<float*>(other), new_refs, new_blocks);
   else
     hipLaunchKernelGGL((k_rebase<double>), grid, block, 0, s, b, static_cast<double*>(cur),
-                       static_cast<double*>(other), new_refs);
+                       static_cast<double*>(other), new_refs, new_blocks);
   return hipGetLastError();
 }
 

@@ -64,7 +64,8 @@ class ChannelParams(ctypes.Structure):
 
 class Options(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("device", ctypes.c_int32),
-                ("use_graph", ctypes.c_int32), ("compute_fp64", ctypes.c_int32)]
+                ("use_graph", ctypes.c_int32), ("compute_fp64", ctypes.c_int32),
+                ("reject_unresolved", ctypes.c_int32)]
 
 
 class EkpnpError(RuntimeError):
@@ -87,6 +88,7 @@ SIGNATURES = [
     ("ekpnp_lattice_params", ctypes.c_int, [_P(ChannelParams), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     ("ekpnp_default_options", None, [_P(Options)]),
+    ("ekpnp_check_params", ctypes.c_int, [_P(ChannelParams), ctypes.c_int, _P(ctypes.c_uint32)]),
     ("ekpnp_create", ctypes.c_int, [_P(ChannelParams), ctypes.c_int, _P(Options), _P(_VP)]),
     ("ekpnp_destroy", None, [_VP]),
     ("ekpnp_last_error", ctypes.c_char_p, [_VP]),
@@ -125,6 +127,7 @@ SIGNATURES = [
     ("ekpnp_steps_done", ctypes.c_longlong, [_VP]),
     ("ekpnp_device_bytes", ctypes.c_size_t, [_VP]),
     ("ekpnp_time_stages", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
+    ("ekpnp_copy_bandwidth", ctypes.c_int, [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, _DP, _DP]),
 ]
 
 
@@ -216,6 +219,49 @@ def lattice_params(nx: int, ny: int, tau: float = 0.8, zeta: float = -1.0, kappa
     return p
 
 
+REBASE_INTERVAL = 4096  # EKPNP_REBASE_INTERVAL
+
+
+def rebase_steps(upto: int):
+    """Steps after init_equilibrium at which FP32 storage re-centres its
+    references inside ekpnp_step (include/ekpnp.h): 8, 16, ..., 4096, then
+    every 4096, up to and including `upto`."""
+    out, last = [], 0
+    while True:
+        last = max(8, 2 * last) if last < REBASE_INTERVAL else last + REBASE_INTERVAL
+        if last > upto:
+            return out
+        out.append(last)
+
+
+def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, grid: int = 1024, device: int = 0) -> dict:
+    """Non-temporal float4 device copy of nbytes (ekpnp_copy_bandwidth):
+    {"GBps": read + write bytes per second / 1e9, "ms_per_copy": ...}."""
+    g = ctypes.c_double(0.0)
+    ms = ctypes.c_double(0.0)
+    rc = lib().ekpnp_copy_bandwidth(int(device), int(nbytes), int(reps), int(grid), ctypes.byref(g),
+                                    ctypes.byref(ms))
+    if rc != OK:
+        raise EkpnpError(rc, "copy bandwidth measurement failed")
+    return {"GBps": g.value, "ms_per_copy": ms.value, "bytes": int(nbytes), "grid": int(grid), "reps": int(reps)}
+
+
+CHECK_DEBYE_UNRESOLVED, CHECK_WALL_DRIFT = 0x1, 0x2
+
+
+def check_params(channels: Sequence[ChannelParams]) -> np.ndarray:
+    """EKPNP_CHECK_* flags per channel (ekpnp_check_params; no device needed):
+    Debye length unresolved (kappa dy > 0.4), ions' wall drift above 0.05."""
+    if isinstance(channels, ChannelParams):
+        channels = [channels]
+    arr = (ChannelParams * len(channels))(*channels)
+    out = (ctypes.c_uint32 * len(channels))()
+    rc = lib().ekpnp_check_params(arr, len(channels), out)
+    if rc < 0:
+        raise EkpnpError(rc, "ekpnp_check_params: no channels")
+    return np.array(out[:], dtype=np.uint32)
+
+
 def _dptr(a: Optional[np.ndarray]):
     if a is None:
         return None
@@ -235,7 +281,7 @@ class Solver:
     """One EK-PNP context: every channel of `channels` advances in lock-step."""
 
     def __init__(self, channels: Sequence[ChannelParams], precision: str = "fp32", device: int = 0,
-                 use_graph: bool = True, compute_fp64: bool = True):
+                 use_graph: bool = True, compute_fp64: bool = True, reject_unresolved: bool = False):
         L = lib()
         if isinstance(channels, ChannelParams):
             channels = [channels]
@@ -247,6 +293,7 @@ class Solver:
         opt.device = device
         opt.use_graph = 1 if use_graph else 0
         opt.compute_fp64 = 1 if compute_fp64 else 0
+        opt.reject_unresolved = 1 if reject_unresolved else 0
         self.precision = precision
         ctx = _VP()
         rc = L.ekpnp_create(arr, len(self.channels), ctypes.byref(opt), ctypes.byref(ctx))
@@ -429,5 +476,6 @@ __all__ = [
     "OK", "ERR_INVALID", "ERR_HIP", "ERR_NOMEM", "ERR_STATE", "ERR_UNSUPPORTED", "ERR_UNSTABLE", "ERR_IO",
     "ChannelParams", "Options", "Solver", "EkpnpError", "lib", "reference_params", "lattice_params",
     "FIELDS", "SPECIES", "RED_NAMES", "COLD_NOISE", "REFERENCE", "PB_PICARD", "POP_FP32", "POP_FP64",
-    "FLAG_ION_DRIFT_APPLIED", "FLAG_G0_FLUID_U", "SIGNATURES", "LIB_PATH",
+    "FLAG_ION_DRIFT_APPLIED", "FLAG_G0_FLUID_U", "SIGNATURES", "LIB_PATH", "REBASE_INTERVAL", "rebase_steps", "copy_bandwidth", "check_params",
+    "CHECK_DEBYE_UNRESOLVED", "CHECK_WALL_DRIFT",
 ]

@@ -328,13 +328,13 @@ def channel_of(zeta, kappa_h, e_x, nx, ny, tau=0.8, diff=0.05) -> ChannelParams:
                           diff=diff)
 
 
-def stable(p: ChannelParams, max_kappa_dy=0.4, max_drift=0.05) -> bool:
-    """Acceptance rule for sampled channels: the Debye length is resolved
-    (kappa dy <= max_kappa_dy) and the Gouy-Chapman wall drift of the ions,
-    D * 2 kappa sinh(|zeta|/2), stays far below the lattice sound speed."""
-    kappa = np.sqrt(2.0 * p.poisson_coef)
-    drift = p.diff_pos * 2.0 * kappa * np.sinh(abs(p.psi_lo) / 2.0)
-    return bool(kappa * p.dy <= max_kappa_dy and drift <= max_drift)
+def stable(p: ChannelParams) -> bool:
+    """Acceptance rule for sampled channels, the library's ekpnp_check_params:
+    the Debye length is resolved (kappa dy <= 0.4) and the Gouy-Chapman wall
+    drift of the ions, K * 2 kappa sinh(|zeta|/2) in lattice units, stays far
+    below the lattice sound speed (<= 0.05)."""
+    from . import check_params
+    return bool(check_params([p])[0] == 0)
 
 
 def sample_channels(n: int, seed: int, nx_choices=(32, 64, 128, 256), ny_choices=None,
@@ -358,6 +358,29 @@ def sample_channels(n: int, seed: int, nx_choices=(32, 64, 128, 256), ny_choices
         else:
             rejected += 1
     return out, rejected
+
+
+def save_net(path: str, net: "MLP", **meta) -> None:
+    """Write a trained network's FP32 master parameters and shape (numpy .npz,
+    no pickled objects) with optional metadata (strings and numbers)."""
+    c = net.cfg
+    np.savez(path, params=net.get_params(), shape=np.array([c.n_in, c.n_hidden, c.n_layers, c.n_out], np.int64),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_net(path: str, max_batch: int = 512, device: int = 0) -> "MLP":
+    """A network written by save_net, on the GPU (numpy load without pickle)."""
+    with np.load(path, allow_pickle=False) as z:
+        n_in, n_hidden, n_layers, n_out = (int(v) for v in z["shape"])
+        params = z["params"]
+    net = MLP(n_in=n_in, n_hidden=n_hidden, n_layers=n_layers, n_out=n_out, max_batch=max_batch, device=device)
+    net.set_params(params)
+    return net
+
+
+def net_meta(path: str) -> dict:
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k].tolist() for k in z.files if k not in ("params",)}
 
 
 def channel_features(ps: Sequence[ChannelParams]) -> np.ndarray:

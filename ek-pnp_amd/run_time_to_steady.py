@@ -234,6 +234,9 @@ def main():
     ap.add_argument("--precision", default="fp32")
     ap.add_argument("--seed", type=int, default=1000)
     ap.add_argument("--save", default="", help="write the trained parameters and the training set (.npz)")
+    ap.add_argument("--save-net", default="",
+                    help="write the trained network alone (surrogate.save_net: parameters, shape, training "
+                         "command) for surrogate.load_net")
     ap.add_argument("--load-data", default="",
                     help="take the training targets and step counts from a --save file of the same "
                          "--train/--seed/--ny/--nx-train/--targets instead of generating them")
@@ -309,6 +312,10 @@ def main():
     }
     if args.save:
         np.savez(args.save, params=net.get_params(), X=X, T=T, steps=tr_steps)
+    if args.save_net:
+        S.save_net(args.save_net, net, command=" ".join(["run_time_to_steady.py"] + sys.argv[1:]),
+                   train_channels=int(good.sum()), ny=list(ny), nx_train=args.nx_train, seed=args.seed,
+                   targets=args.targets, epochs=args.epochs, val_mse=float(hist[-1][2]))
     print(json.dumps(out), flush=True)
 
 

@@ -58,7 +58,8 @@ enum {
 
 /* ---- population storage ------------------------------------------------ */
 enum {
-  EKPNP_POP_FP32 = 0, /* FP32 deviations from w_i*(rho, c+, c-)_ref(y)    */
+  EKPNP_POP_FP32 = 0, /* FP32 deviations from w_i*(rho, c+, c-)_ref(x, y)
+                         (one reference per 64-node block of a row)        */
   EKPNP_POP_FP64 = 1  /* FP64 full populations (the reference's storage)   */
 };
 
@@ -107,7 +108,8 @@ enum {
   EKPNP_RED_CHARGE = 3,    /* sum (c+ - c-) over interior rows              */
   EKPNP_RED_DUX2 = 4,      /* sum (ux - ux_prev)^2 over interior rows       */
   EKPNP_RED_UX2 = 5,       /* sum ux^2 over interior rows                   */
-  EKPNP_RED_NONFINITE = 6, /* count of non-finite rho/ux/c values           */
+  EKPNP_RED_NONFINITE = 6, /* count of nodes with a non-finite rho, ux, uy,
+                              c+, c- or psi                                  */
   EKPNP_RED_RESIDUAL = 7,  /* sqrt(DUX2/UX2)/interval (0 before a snapshot) */
   EKPNP_RED_COUNT = 8
 };
@@ -158,8 +160,35 @@ typedef struct ekpnp_options {
   int32_t precision; /* EKPNP_POP_FP32 or EKPNP_POP_FP64 */
   int32_t device;    /* HIP device ordinal */
   int32_t use_graph; /* capture ekpnp_step chunks in hipGraphs */
-  int32_t compute_fp64; /* FP32 storage only: collide in FP64 arithmetic */
+  int32_t compute_fp64; /* FP32 storage only: collide in FP64 arithmetic (the
+                           default, 1).  0 collides in FP32 arithmetic, which
+                           does NOT hold the 1e-5 relative-L2 accuracy of the
+                           default: u_x drifts ~1e-4 from FP64 over 20 000
+                           steps of a 64 x 128 channel (the drift velocity is
+                           a small difference of O(1) population sums); for
+                           throughput comparisons only */
+  int32_t reject_unresolved; /* 1: ekpnp_create fails with
+                           EKPNP_ERR_UNSTABLE for a channel that
+                           ekpnp_check_params flags (default 0: accepted,
+                           as the reference accepts any parameters) */
 } ekpnp_options;
+
+/* ---- parameter check ------------------------------------------------------ */
+/* Flags of ekpnp_check_params */
+#define EKPNP_CHECK_DEBYE_UNRESOLVED 0x1u /* kappa dy > 0.4                      */
+#define EKPNP_CHECK_WALL_DRIFT 0x2u       /* Gouy-Chapman wall drift of the ions
+                                             > 0.05 lattice speed (dx/dt)        */
+/* Resolution and stability rule for each channel, without a device:
+ * kappa = sqrt(2 poisson_coef c_bulk inv_thermal_voltage) (linearised
+ * Poisson-Boltzmann), the Debye length resolved (kappa dy <= 0.4) and the
+ * ions' drift in the Gouy-Chapman wall field, max|K| (2 kappa / inv_vt)
+ * sinh(inv_vt max|psi_wall| / 2), at most 0.05 dx/dt.  Channels outside the
+ * rule need not fail, but strongly charged thin-layer channels past it have
+ * gone non-finite (the batched sweep's sampler redraws them).  flags_out
+ * (may be NULL, else n values) receives EKPNP_CHECK_* per channel; returns
+ * the number of channels flagged, or EKPNP_ERR_INVALID.  No counterpart in
+ * the reference. */
+int ekpnp_check_params(const ekpnp_channel_params* channels, int n_channels, uint32_t* flags_out);
 
 typedef struct ekpnp_ctx ekpnp_ctx;
 
@@ -219,14 +248,20 @@ int ekpnp_fast_poisson(ekpnp_ctx* ctx);
  * Synchronises every check_every sweeps. */
 int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double tol, int max_iter,
                             int check_every, int* iters_out, double* change_out);
-/* With FP32 storage, ekpnp_step re-centres the storage reference every
- * EKPNP_REBASE_INTERVAL steps (see ekpnp_rebase), inside a long call too. */
+/* With FP32 storage, ekpnp_step re-centres the storage references (see
+ * ekpnp_rebase) 8, 16, 32, ... steps after the last re-centring, the
+ * interval doubling up to EKPNP_REBASE_INTERVAL and staying there (counted
+ * from init_equilibrium), inside a long call too: the re-centring steps
+ * depend on the step count only, not on how the run is cut into calls. */
 int ekpnp_step(ekpnp_ctx* ctx, int nsteps);
-/* FP32 storage keeps each population as a deviation from w_i times a per-row
- * reference (rho, c+, c-)(y).  This re-centres the references on the current
- * row means over x; the state it represents is unchanged up to one FP32
- * rounding.  No-op for FP64 storage.  No counterpart in the reference (its
- * populations are absolute FP64 values). */
+/* FP32 storage keeps each population as a deviation from w_i times a
+ * reference (rho, c+, c-)(x, y), the mean over the node's 64-node block of its
+ * row (the whole row when nx < 64), plus w_i c_ix/(cs^2 CFL) times the row's
+ * mean x-flux.  This re-centres the references on the current fields; the
+ * state it represents is unchanged up to one FP32 rounding.  Channels frozen
+ * by a running ekpnp_run_to_residual keep theirs.  No-op for FP64 storage.
+ * No counterpart in the reference (its populations are absolute FP64
+ * values). */
 int ekpnp_rebase(ekpnp_ctx* ctx);
 /* Steps until every channel's residual sqrt(sum (ux-ux_prev)^2 / sum ux^2)
  * / check_every over interior rows is <= tol, or max_steps.  check_every
@@ -320,6 +355,15 @@ size_t ekpnp_device_bytes(const ekpnp_ctx* ctx);
  * on the tridiagonal launch (ekpnp_get_poisson_plan), [0] covers the other
  * rows and [2] includes the wall rows. */
 int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out);
+/* Measurement: a non-temporal float4 device copy of `bytes` bytes (a
+ * multiple of 16) on `device`, grid work-groups of 256 lanes striding over
+ * the buffer, timed with HIP events over reps copies after one untimed one:
+ * *gbps_out = 2 bytes / (time per copy) (read + write), *ms_out (may be NULL)
+ * the time per copy.  The streaming ceiling the LBM kernel's HBM rate is set
+ * against, and the kernel the FETCH_SIZE / WRITE_SIZE counters are
+ * calibrated on.  Allocates and frees its two buffers.  No counterpart in
+ * the reference. */
+int ekpnp_copy_bandwidth(int device, size_t bytes, int reps, int grid, double* gbps_out, double* ms_out);
 
 #ifdef __cplusplus
 }

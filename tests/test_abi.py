@@ -105,3 +105,37 @@ def test_python_defaults_match_the_c_defaults():
     sig = inspect.signature(ekpnp.Solver.__init__).parameters
     assert o.compute_fp64 == 1 and sig["compute_fp64"].default is True
     assert o.use_graph == 1 and sig["use_graph"].default is True
+
+
+def test_check_params_flags_unresolved_and_fast_drift_channels():
+    # kappa = sqrt(2 pc c_bulk inv_vt) = kappa_h / h in lattice units; the
+    # drift is K (2 kappa) sinh(|zeta|/2) for psi in kT/e
+    import math
+    ok = ekpnp.lattice_params(64, 128, zeta=-1.0, kappa_h=8.0)            # kappa dy 0.126
+    debye = ekpnp.lattice_params(64, 40, zeta=-0.5, kappa_h=9.0)          # kappa dy 0.46, drift 0.012
+    drift = ekpnp.lattice_params(64, 200, zeta=-4.0, kappa_h=30.0)        # kappa dy 0.30, drift 0.11
+    ref = ekpnp.reference_params()                                        # kappa dy 0.11 (SI units)
+    flags = ekpnp.check_params([ok, debye, drift, ref])
+    assert flags.tolist() == [0, ekpnp.CHECK_DEBYE_UNRESOLVED, ekpnp.CHECK_WALL_DRIFT, 0]
+    kappa = 30.0 / (0.5 * 199)
+    assert 0.05 < 0.05 * 2 * kappa * math.sinh(2.0) and kappa < 0.4
+    # the count and argument checks of the C entry point
+    arr = (ekpnp.ChannelParams * 4)(ok, debye, drift, ref)
+    assert ekpnp.lib().ekpnp_check_params(arr, 4, None) == 2
+    assert ekpnp.lib().ekpnp_check_params(arr, 0, None) == ekpnp.ERR_INVALID
+
+
+def test_create_rejects_flagged_channels_on_request_before_touching_a_device():
+    lib = ekpnp.lib()
+    ctx = ctypes.c_void_p()
+    good = ekpnp.lattice_params(64, 128, zeta=-1.0, kappa_h=8.0)
+    bad = ekpnp.lattice_params(64, 40, zeta=-0.5, kappa_h=9.0)
+    arr = (ekpnp.ChannelParams * 2)(good, bad)
+    o = ekpnp.Options()
+    lib.ekpnp_default_options(ctypes.byref(o))
+    assert o.reject_unresolved == 0
+    o.reject_unresolved = 1
+    assert lib.ekpnp_create(arr, 2, ctypes.byref(o), ctypes.byref(ctx)) == ekpnp.ERR_UNSTABLE
+    msg = lib.ekpnp_create_error()
+    assert b"channel 1" in msg and b"kappa dy" in msg, msg
+    assert not ctx.value

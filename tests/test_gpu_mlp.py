@@ -29,7 +29,9 @@ def test_forward_matches_reference(batch):
     yr, _ = R.forward(p, x, shp, emulate=True)
     e = _rel(y, yr)
     print("forward rel", batch, e, flush=True)
-    assert e < 5e-3
+    # same BF16 roundings as the reference, FP32 sums in another order:
+    # measured ~5e-8
+    assert e < 1e-5
 
 
 def test_gradients_match_reference():
@@ -40,13 +42,13 @@ def test_gradients_match_reference():
     t = rng.standard_normal((80, 128)) * 0.5
     loss, g = m.loss_grad(x, t)
     lr, gr = R.loss_grad(p, x, t, shp, emulate=True)
-    assert abs(loss - lr) <= 1e-3 * lr
+    assert abs(loss - lr) <= 1e-6 * lr
     off = 0
     for li, (o, i) in enumerate(shp):
         for name, n in (("W", o * i), ("b", o)):
             e = _rel(g[off:off + n], gr[off:off + n])
             print(f"grad layer {li} {name} rel {e:.2e}", flush=True)
-            assert e < 2e-2, (li, name, e)
+            assert e < 1e-5, (li, name, e)  # measured 5e-8 .. 7e-7
             off += n
     # the emulated reference stays close to the pure FP64 gradient
     _, g64 = R.loss_grad(p, x, t, shp, emulate=False)
@@ -63,7 +65,11 @@ def test_full_size_network_forward():
         m.forward(x[:, :4])
     e = _rel(m.forward(x), R.forward(p, x, shp, emulate=True)[0])
     print("full forward rel", e, flush=True)
-    assert e < 5e-3
+    # four 256-wide BF16 layers: where an FP32 sum lands next to a BF16
+    # rounding boundary, the different summation order can round the other
+    # way (a 2^-8 relative step of that activation), which the next layers
+    # carry on; measured 7.6e-5 over the whole output
+    assert e < 5e-4
 
 
 def test_training_reduces_loss_on_smooth_targets():

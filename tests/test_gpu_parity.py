@@ -485,6 +485,71 @@ def test_heterogeneous_wall_charge_parity():
                 assert errs[f] < tol, (f, errs[f])
 
 
+def _zeta_walls(nx, seed=4):
+    rng = np.random.default_rng(seed)
+    a, phi = rng.uniform(0, 0.5, 4), rng.uniform(0, 2 * np.pi, 4)
+    x = np.arange(nx)
+    z = -1.5 + sum(a[k] * np.sin(2 * np.pi * (k + 1) * x / nx + phi[k]) for k in range(4))
+    return dict(psi_lo=z, psi_hi=z[::-1].copy(), cpos_lo=np.exp(-z), cpos_hi=np.exp(-z[::-1]),
+                cneg_lo=np.exp(z), cneg_hi=np.exp(z[::-1]))
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("nx,ny,nsteps", [(64, 48, 16000), (256, 32, 8000)])
+def test_heterogeneous_wall_charge_fp32_long_horizon(nx, ny, nsteps):
+    # FP32 storage on zeta(x) walls against the FP64 oracle over many steps,
+    # through the early re-centrings of the cold start and several at the
+    # 4096-step interval; 256 columns hold four 64-node reference blocks per
+    # row (64 columns: one)
+    print(f"case zeta(x) fp32 {nx}x{ny}, {nsteps} steps", flush=True)
+    p = ekpnp.lattice_params(nx, ny, tau=0.8, zeta=-1.5, kappa_h=6.0, e_x=1e-4)
+    walls = _zeta_walls(nx)
+    s = ekpnp.Solver([p], precision="fp32")
+    s.set_wall_profile(0, **walls)
+    s.initialization(ekpnp.COLD_NOISE, 4)
+    s.init_equilibrium()
+    o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
+    o.set_wall_profile(**walls)
+    o.initialization(ekpnp.COLD_NOISE, 4, 0)
+    o.init_equilibrium()
+    for n in (nsteps // 4, nsteps):
+        s.step(n - s.steps_done)
+        o.step(n - o.steps)
+        errs = field_errors(s, 0, o)
+        _report(f"zeta(x) fp32 {nx}x{ny} n={n}", errs)
+        for f in ("ux", "uy", "cpos", "cneg", "psi"):
+            assert errs[f] < 1e-5, (n, f, errs[f])
+
+
+def test_run_to_residual_frozen_fp32_channel_keeps_its_references():
+    # FP32 storage: a channel frozen by run_to_residual keeps its lattice and
+    # its storage references through the re-centrings the other channel's
+    # steps bring; here the run starts at an odd step (so the frozen state
+    # sits in the other lattice half of the time) and re-centres while the
+    # fast channel is frozen.  Its state at the end must equal a run of that
+    # channel alone, bit for bit.
+    pa = ekpnp.lattice_params(8, 12, tau=1.0, zeta=-0.5, kappa_h=1.5, e_x=1e-4)
+    pb = ekpnp.lattice_params(64, 40, tau=1.0, zeta=-0.5, kappa_h=1.5, e_x=1e-4)
+    tol, every = 1e-7, 20
+    both = ekpnp.Solver([pa, pb], precision="fp32")
+    single = ekpnp.Solver([pa], precision="fp32")
+    for s in (both, single):
+        s.initialization(ekpnp.COLD_NOISE, 9)
+        s.init_equilibrium()
+        s.step(1)
+    steps = both.run_to_residual(tol, every, 400000)
+    s1 = single.run_to_residual(tol, every, 400000)
+    print("steps", steps, s1, flush=True)
+    assert steps[0] > 0 and steps[1] > steps[0] and s1[0] == steps[0]
+    # re-centrings fell between the freeze and the end of the call
+    between = [r for r in ekpnp.rebase_steps(1 + steps[1]) if 1 + steps[0] < r <= 1 + steps[1]]
+    assert between, "no re-centring while the channel was frozen"
+    for f in ("rho", "ux", "uy", "cpos", "cneg", "psi"):
+        assert np.array_equal(both.get_field(0, f), single.get_field(0, f)), f
+    for sp in ("f", "h", "g"):
+        assert np.array_equal(both.get_populations(0, sp), single.get_populations(0, sp)), sp
+
+
 def test_nonfinite_population_is_reported_for_its_channel():
     # failure injection from the host: a NaN planted in channel 1's populations
     # must stop run_to_residual with ERR_UNSTABLE naming channel 1, while
