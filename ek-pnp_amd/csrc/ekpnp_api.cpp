@@ -703,6 +703,12 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     g_create_error = std::string("hipSetDevice: ") + hipGetErrorString(he);
     return EKPNP_ERR_HIP;
   }
+  // the device's copy of the allocation-free reductions' scratch array
+  he = noalloc_scratch_init(o.device);
+  if (he != hipSuccess) {
+    g_create_error = std::string("scratch address: ") + hipGetErrorString(he);
+    return EKPNP_ERR_HIP;
+  }
   ekpnp_ctx* c = new ekpnp_ctx();
   c->opt = o;
   c->nchan = n_channels;
@@ -1306,6 +1312,31 @@ int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
   HIP_OR_FAIL(ctx, launch_reductions(ctx->sb, 1, ctx->stream));
   HIP_OR_FAIL(ctx, hipMemcpyAsync(out, ctx->d_red, 8 * (size_t)ctx->nchan * sizeof(double),
                                   hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  finish_reductions(ctx, out);
+  return EKPNP_OK;
+}
+
+int ekpnp_reductions_noalloc(ekpnp_ctx* ctx, double* out) {
+  if (!ctx || !out) return EKPNP_ERR_INVALID;
+  const long long need = 8LL * ((long long)ctx->lbm_tiles.size() + ctx->nchan);
+  if (need > NOALLOC_SCRATCH_DOUBLES)
+    return fail(ctx, EKPNP_ERR_UNSUPPORTED,
+                "ekpnp_reductions_noalloc: 8 x (LBM tiles + channels) = " + std::to_string(need) +
+                    " doubles exceed its scratch of " + std::to_string(NOALLOC_SCRATCH_DOUBLES) +
+                    " (contexts of up to about 33 million nodes)");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  double* ptr = noalloc_scratch(ctx->opt.device);
+  if (!ptr) return fail(ctx, EKPNP_ERR_STATE, "ekpnp_reductions_noalloc: scratch address not set up");
+  // partial sums of every tile, then the per-channel records, in the
+  // scratch array; both kernels write every slot they later read, so
+  // nothing is cleared first
+  StepBuffers b = ctx->sb;
+  b.partials = ptr;
+  b.red_out = ptr + 8LL * (long long)ctx->lbm_tiles.size();
+  HIP_OR_FAIL(ctx, launch_reductions(b, 1, ctx->stream));
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(out, b.red_out, 8 * (size_t)ctx->nchan * sizeof(double), hipMemcpyDeviceToHost,
+                                  ctx->stream));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   finish_reductions(ctx, out);
   return EKPNP_OK;

@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/ekpnp.h"
+
 namespace ekpnp {
 
 // Wall arrays are stored as 6 planes of length total_nx (sum of nx over the
@@ -163,6 +165,13 @@ hipError_t launch_sor_sweep(const StepBuffers& b, int chan, int nx, int ny, int 
 hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s);
 // psi wall rows of channel chan from the wall potentials
 hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStream_t s);
+// Allocation-free reductions: a file-scope device array of
+// NOALLOC_SCRATCH_DOUBLES doubles per GPU; noalloc_scratch_init looks up its
+// address on `device` (which must be current) once, noalloc_scratch returns
+// it (nullptr before the lookup)
+constexpr long long NOALLOC_SCRATCH_DOUBLES = EKPNP_NOALLOC_SCRATCH_DOUBLES;  // 8 MiB
+hipError_t noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
 // dst = src, n_float4 16-byte values, non-temporal, grid work-groups of 256 lanes
 hipError_t launch_copy_nt(const void* src, void* dst, long long n_float4, int grid, hipStream_t s);
 

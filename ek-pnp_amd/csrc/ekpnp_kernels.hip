@@ -33,6 +33,7 @@
 // Right after init_equilibrium or set_populations the lattice holds the
 // pre-collision populations themselves, unshifted ("Q"), and the first step
 // reads them locally (FROM_Q).
+#include <mutex>
 #include <type_traits>
 
 #include "ekpnp_internal.h"
@@ -2447,6 +2448,42 @@ hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s) 
 hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStream_t s) {
   hipLaunchKernelGGL(k_psi_wall_rows, dim3((nx + 255) / 256), dim3(256), 0, s, b, chan);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free reductions: a file-scope device array, one
+// copy per GPU.  Its host-side name is not a device address: every kernel
+// argument, copy or fill uses the address hipGetSymbolAddress returns for
+// the current device, looked up once per device (noalloc_scratch_init, at
+// context creation) and kept here.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+  void* ptr = nullptr;
+  e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return hipSuccess;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 hipError_t launch_reductions(const StepBuffers& b, int snapshot, hipStream_t s) {

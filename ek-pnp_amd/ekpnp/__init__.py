@@ -110,6 +110,7 @@ SIGNATURES = [
     ("ekpnp_get_populations", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
     ("ekpnp_set_populations", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _DP]),
     ("ekpnp_reductions", ctypes.c_int, [_VP, _DP]),
+    ("ekpnp_reductions_noalloc", ctypes.c_int, [_VP, _DP]),
     ("ekpnp_synchronize", ctypes.c_int, [_VP]),
     ("ekpnp_save_data_end", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_char_p]),
     ("ekpnp_read_data", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_char_p, _DP]),
@@ -400,6 +401,13 @@ class Solver:
     def reductions(self) -> np.ndarray:
         out = np.empty((len(self.channels), 8), dtype=np.float64)
         self._check(lib().ekpnp_reductions(self._ctx, _dptr(out)))
+        return out
+
+    def reductions_noalloc(self) -> np.ndarray:
+        """reductions() through ekpnp_reductions_noalloc (no device
+        allocation; the library's fixed scratch array)."""
+        out = np.empty((len(self.channels), 8), dtype=np.float64)
+        self._check(lib().ekpnp_reductions_noalloc(self._ctx, _dptr(out)))
         return out
 
     # -- files and diagnostics ----------------------------------------------

@@ -281,6 +281,20 @@ int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species,
 /* Per-channel FP64 reductions, out[channel*EKPNP_RED_COUNT + EKPNP_RED_*];
  * also snapshots ux for the next residual. */
 int ekpnp_reductions(ekpnp_ctx* ctx, double* out);
+/* ekpnp_reductions without any device allocation or allocating library call:
+ * no counterpart in the reference; for callers that cannot allocate (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The same kernels, the same results and the same ux snapshot,
+ * with the per-tile partial sums and the per-channel records in a
+ * library-wide device array of EKPNP_NOALLOC_SCRATCH_DOUBLES doubles (8 MiB;
+ * one copy per GPU, held until the process ends) instead of the context's
+ * buffers: contexts whose 8 x (LBM work-groups + channels) exceed it (about
+ * 33 million nodes: one work-group per 256 nodes of a channel) get
+ * EKPNP_ERR_UNSUPPORTED.  Calls on one GPU share the array: calls that
+ * overlap on one GPU (two host threads, two contexts' streams) race and must
+ * be serialised by the caller.  Synchronises the context's stream. */
+#define EKPNP_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+int ekpnp_reductions_noalloc(ekpnp_ctx* ctx, double* out);
 int ekpnp_synchronize(ekpnp_ctx* ctx);
 
 /* ---- files and diagnostics (host formatting of one channel's fields) -------- */
