@@ -245,9 +245,10 @@ def test_fp32_deviation_storage_64x128():
 
 
 def test_fp32_arithmetic_variant_is_bounded():
-    # FP32 collision arithmetic (compute_fp64=False) is an option, not the
-    # default: its u_x error grows to ~1e-4 over 20000 steps (the small
-    # drift velocity is a difference of O(1) population sums), bounded here
+    # FP32 collision arithmetic (compute_fp64=False) is an option for
+    # throughput comparisons, not the default, and does NOT meet the 1e-5 bar:
+    # its u_x error grows to ~6e-5 over 20000 steps (the small drift velocity
+    # is a difference of O(1) population sums; measured 6.0e-5), bounded here
     errs = _fp32_errors_64x128(False)
     for f in ("cpos", "cneg"):
         assert errs[f] < 1e-5, (f, errs[f])
@@ -588,10 +589,11 @@ def test_extreme_shapes(nx, ny, precision):
         assert_parity(s, 0, o, FP64_TOL, tag=f"{nx}x{ny}")
     else:
         # FP32 deviation storage: on two interior rows u_x (~3e-4 here) is a
-        # small difference of O(1) populations rounded to FP32, so its
-        # relative error after 30 steps is ~1.5e-5 (the same bits with either
-        # LBM kernel; FP64 storage agrees to 4e-13); 5e-5 for u_x, 1e-5 else
+        # small difference of O(1) populations rounded to FP32; the
+        # re-centrings at steps 8 and 16 keep it within the bar (measured
+        # 6.9e-6 at 256 x 4, 3.3e-6 at 24 x 7); u_y of these x-uniform
+        # channels is noise-level after 30 steps, its relative error meaningless
         errs = field_errors(s, 0, o)
         _report(f"fp32 {nx}x{ny}", errs)
         for f in ("ux", "cpos", "cneg", "psi"):
-            assert errs[f] < (5e-5 if f == "ux" else 1e-5), (f, errs[f])
+            assert errs[f] < 1e-5, (f, errs[f])

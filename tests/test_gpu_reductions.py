@@ -57,24 +57,26 @@ def test_reductions_match_numpy_sums(precision):
             assert red[c, IDX["nonfinite"]] == 0
             res = np.sqrt(dux2 / want["ux2"]) / interval
             assert _close(red[c, IDX["residual"]], res, 1e-10), (c, red[c, IDX["residual"]], res)
-            assert red0[c, IDX["residual"]] > 0.0  # the first call's interval is 300 steps
+            # the first call has no earlier snapshot: residual 0 by definition
+            assert red0[c, IDX["residual"]] == 0.0 and red[c, IDX["residual"]] > 0.0
         s.close()
 
 
 def test_nonfinite_count_covers_every_field():
-    # a NaN planted in one population of channel 1: the collide-stream makes
-    # that node's rho and u non-finite (one node counted); the Poisson solve
-    # then spreads it through psi to every interior node of the channel (the
-    # wall rows keep their wall potentials); channel 0 stays clean
+    # a NaN planted in one positive-ion population of channel 1: the
+    # collide-stream makes that node's c+, its charge and (through the body
+    # force) u non-finite (one node counted); the Poisson solve then spreads
+    # the charge through psi to every interior node of the channel (the wall
+    # rows keep their wall potentials); channel 0 stays clean
     ps = [ekpnp.lattice_params(16, 20, tau=0.8, zeta=-1.0, kappa_h=3.0, e_x=1e-4),
           ekpnp.lattice_params(64, 24, tau=0.8, zeta=-1.0, kappa_h=3.0, e_x=1e-4)]
     s = ekpnp.Solver(ps, precision="fp64")
     s.initialization(ekpnp.COLD_NOISE, 1)
     s.init_equilibrium()
     s.step(4)
-    f = s.get_populations(1, "f")
-    f[0, 10, 7] = np.nan
-    s.set_populations(1, "f", f)
+    h = s.get_populations(1, "h")
+    h[0, 10, 7] = np.nan
+    s.set_populations(1, "h", h)
     s.stream_collide_save()
 
     def bad_nodes():
