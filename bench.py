@@ -278,11 +278,11 @@ def main():
         # the LBM stage is the interior-row launch; rows 0 and ny-1 are
         # updated inside the tridiagonal launch
         stage_nodes = sum(p.nx * (p.ny - 2) if p.nx % 64 == 0 else p.nx * p.ny for p in params)
-        kernel = ("k_lbm_rows<false,false>: fused collide-stream of the interior rows "
+        kernel = ("k_lbm_rows<false>: fused collide-stream of the interior rows "
                   "(the two wall rows ride on the tridiagonal launch)")
     else:
         stage_nodes = nodes
-        kernel = ("LBM stage: k_lbm_rows<false,false> (interior rows) + k_lbm_rows<false,true> (wall rows), "
+        kernel = ("LBM stage: k_lbm_rows<false> (interior rows) + k_lbm_rows<true> (wall rows), "
                   "fused collide-stream")
     achieved = lbm_b * stage_nodes / (st["lbm"] * 1e-3) / 1e9
     out = {
