@@ -20,9 +20,22 @@
 #include "../../include/ekpnp_seed.h"
 #include "ekpnp_internal.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 using namespace ekpnp;
 
 namespace {
+
+// A roctx range over one call of an entry point: the host-side span of the
+// call (kernel launches, graph replays, copies) as a named range under
+// rocprofv3 --marker-trace; with no tool attached a push/pop pair is two
+// calls that return at once.
+struct ApiRange {
+  explicit ApiRange(const char* name) { roctxRangePushA(name); }
+  ~ApiRange() { roctxRangePop(); }
+  ApiRange(const ApiRange&) = delete;
+  ApiRange& operator=(const ApiRange&) = delete;
+};
 
 const double W[9] = {4.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0, 1.0 / 9.0,
                      1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0, 1.0 / 36.0};
@@ -32,6 +45,7 @@ const int CXH[9] = {0, 1, 0, -1, 0, 1, -1, -1, 1};
 // BLK_STRIDE floats, X over each 64-node block of the row
 constexpr int kRefStride = 8;
 constexpr int kMaxTriChunks = TRI_WG / 2;  // the plan keeps G = TRI_WG / C >= 2 modes per work-group
+constexpr int kDftLds = 32768;             // LDS bytes of a direct-DFT tile
 
 // stored value + ref[d][y][x] = population (FP32 storage; 0 for FP64)
 std::vector<double> ref_offsets(ekpnp_ctx* c, int channel, int species, hipError_t* err);
@@ -95,6 +109,7 @@ struct ekpnp_ctx {
   int2* d_row_tiles = nullptr;
   double* d_partials = nullptr;
   double* d_red = nullptr;
+  double* d_diag = nullptr;      // diagnostics: 2 x DIAG_MAX_WG partials, then umax and the current sum
   double* d_cprime = nullptr;    // chunk-entry Thomas coefficients (Poisson plan)
   long long total_cprime = 0;    // its capacity: kMaxTriChunks x nk per channel
   size_t dev_bytes = 0;
@@ -269,7 +284,7 @@ void release(ekpnp_ctx* c) {
                   c->d_tri_tiles, c->d_first_tile, c->d_rho_e, c->d_psi, c->d_macro,
                   c->d_walls, c->d_tw, c->d_spec, c->d_ux_prev, c->d_partials, c->d_red,
                   c->d_refs, c->d_refs_new, c->d_row_tiles, c->d_cprime, c->d_lbm_split,
-                  c->d_blocks, c->d_blocks_new};
+                  c->d_blocks, c->d_blocks_new, c->d_diag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -306,7 +321,12 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
         lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(padded * sizeof(double2)));
         lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((padded + d.nx / 4) * sizeof(double2)));
       } else {
-        lds_dft = std::max<unsigned>(lds_dft, (unsigned)std::max(d.nx * sizeof(double), d.nk * sizeof(double2)));
+        // direct-DFT tiles: as many whole rows as fit fft_elems values and
+        // kDftLds bytes of LDS (a row of nx | 1 right-hand-side values, or
+        // its nk spectrum values)
+        const int per_row = (int)std::max((d.nx | 1) * sizeof(double), d.nk * sizeof(double2));
+        rows = std::max(1, std::min({M, fft_elems / d.nx, kDftLds / per_row}));
+        lds_dft = std::max<unsigned>(lds_dft, (unsigned)(rows * per_row));
       }
       d.fft_rows = rows;
       c->ctiles[i].fft_first = (int)c->fft_tiles.size();
@@ -664,6 +684,7 @@ const char* ekpnp_last_error(const ekpnp_ctx* ctx) { return ctx ? ctx->err.c_str
 
 int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekpnp_options* opt,
                  ekpnp_ctx** out) {
+  const ApiRange range_("ekpnp_create");
   g_create_error.clear();
   if (!out) return EKPNP_ERR_INVALID;
   *out = nullptr;
@@ -840,6 +861,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     if ((e = dalloc(c, &c->d_row_tiles, c->row_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_partials, 8 * c->lbm_tiles.size())) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_red, 8 * (size_t)n_channels)) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_diag, 2 * (size_t)DIAG_MAX_WG + 2)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_cprime, c->total_cprime)) != hipSuccess) return e;
     if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return e;
     return hipSuccess;
@@ -927,6 +949,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
 }
 
 void ekpnp_destroy(ekpnp_ctx* ctx) {
+  const ApiRange range_("ekpnp_destroy");
   if (!ctx) return;
   (void)hipSetDevice(ctx->opt.device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
@@ -956,6 +979,7 @@ int ekpnp_set_wall_profile(ekpnp_ctx* ctx, int channel, const double* psi_lo, co
 
 int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho, const double* ux,
                      const double* uy, const double* cpos, const double* cneg, const double* psi) {
+  const ApiRange range_("ekpnp_set_fields");
   int rc = check_channel(ctx, channel);
   if (rc) return rc;
   if (!rho || !ux || !uy || !cpos || !cneg) return fail(ctx, EKPNP_ERR_INVALID, "null field");
@@ -982,6 +1006,7 @@ int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho, const doubl
 }
 
 int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
+  const ApiRange range_("ekpnp_initialization");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (mode != EKPNP_INIT_COLD_NOISE && mode != EKPNP_INIT_REFERENCE && mode != EKPNP_INIT_PB_PICARD)
     return fail(ctx, EKPNP_ERR_INVALID, "unknown initialization mode");
@@ -1032,6 +1057,7 @@ int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
 }
 
 int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
+  const ApiRange range_("ekpnp_init_equilibrium");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "init_equilibrium before initialization/set_fields");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
@@ -1056,6 +1082,7 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
 }
 
 int ekpnp_stream_collide_save(ekpnp_ctx* ctx) {
+  const ApiRange range_("ekpnp_stream_collide_save");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
@@ -1065,6 +1092,7 @@ int ekpnp_stream_collide_save(ekpnp_ctx* ctx) {
 }
 
 int ekpnp_fast_poisson(ekpnp_ctx* ctx) {
+  const ApiRange range_("ekpnp_fast_poisson");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "no fields");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
@@ -1074,6 +1102,7 @@ int ekpnp_fast_poisson(ekpnp_ctx* ctx) {
 
 int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double tol, int max_iter, int check_every,
                             int* iters_out, double* change_out) {
+  const ApiRange range_("ekpnp_poisson_iterative");
   if (!ctx || !iters_out || !change_out) return EKPNP_ERR_INVALID;
   if (channel < 0 || channel >= ctx->nchan) return fail(ctx, EKPNP_ERR_INVALID, "channel out of range");
   if (!(omega > 0.0 && omega < 2.0)) return fail(ctx, EKPNP_ERR_INVALID, "omega must lie in (0, 2)");
@@ -1109,6 +1138,7 @@ int ekpnp_poisson_iterative(ekpnp_ctx* ctx, int channel, double omega, double to
 }
 
 int ekpnp_rebase(ekpnp_ctx* ctx) {
+  const ApiRange range_("ekpnp_rebase");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations: call init_equilibrium first");
   ctx->rebase_step = ctx->steps;
@@ -1128,6 +1158,7 @@ int ekpnp_rebase(ekpnp_ctx* ctx) {
 }
 
 int ekpnp_step(ekpnp_ctx* ctx, int nsteps) {
+  const ApiRange range_("ekpnp_step");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (nsteps < 0) return fail(ctx, EKPNP_ERR_INVALID, "negative step count");
   if (nsteps == 0) return EKPNP_OK;
@@ -1186,6 +1217,7 @@ int ekpnp_synchronize(ekpnp_ctx* ctx) {
 }
 
 int ekpnp_get_field(ekpnp_ctx* ctx, int channel, int field, double* host_out) {
+  const ApiRange range_("ekpnp_get_field");
   int rc = check_channel(ctx, channel);
   if (rc) return rc;
   if (!host_out) return fail(ctx, EKPNP_ERR_INVALID, "null output");
@@ -1225,6 +1257,7 @@ int ekpnp_get_field(ekpnp_ctx* ctx, int channel, int field, double* host_out) {
 }
 
 int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host_out) {
+  const ApiRange range_("ekpnp_get_populations");
   int rc = check_channel(ctx, channel);
   if (rc) return rc;
   if (species < 0 || species > 2 || !host_out) return fail(ctx, EKPNP_ERR_INVALID, "bad species or output");
@@ -1257,6 +1290,7 @@ int ekpnp_get_populations(ekpnp_ctx* ctx, int channel, int species, double* host
 }
 
 int ekpnp_set_populations(ekpnp_ctx* ctx, int channel, int species, const double* host_in) {
+  const ApiRange range_("ekpnp_set_populations");
   int rc = check_channel(ctx, channel);
   if (rc) return rc;
   if (species < 0 || species > 2 || !host_in) return fail(ctx, EKPNP_ERR_INVALID, "bad species or input");
@@ -1307,6 +1341,7 @@ static void finish_reductions(ekpnp_ctx* ctx, double* out) {
 }
 
 int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
+  const ApiRange range_("ekpnp_reductions");
   if (!ctx || !out) return EKPNP_ERR_INVALID;
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, launch_reductions(ctx->sb, 1, ctx->stream));
@@ -1318,6 +1353,7 @@ int ekpnp_reductions(ekpnp_ctx* ctx, double* out) {
 }
 
 int ekpnp_reductions_noalloc(ekpnp_ctx* ctx, double* out) {
+  const ApiRange range_("ekpnp_reductions_noalloc");
   if (!ctx || !out) return EKPNP_ERR_INVALID;
   const long long need = 8LL * ((long long)ctx->lbm_tiles.size() + ctx->nchan);
   if (need > NOALLOC_SCRATCH_DOUBLES)
@@ -1342,7 +1378,33 @@ int ekpnp_reductions_noalloc(ekpnp_ctx* ctx, double* out) {
   return EKPNP_OK;
 }
 
+}  // extern "C"
+
+int ekpnp::channel_diagnostics(ekpnp_ctx* ctx, int channel, double* umax, double* current_sum) {
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ChanDesc& d = ctx->desc[channel];
+  const ekpnp_channel_params& p = ctx->params[channel];
+  const long long nn = (long long)d.nx * d.ny;
+  const int grid = (int)std::min<long long>(DIAG_MAX_WG, std::max<long long>(1, (nn + 4095) / 4096));
+  const double* uy = ctx->d_macro + M_UY * ctx->total_nodes + d.fld_off;
+  const double* cp = ctx->d_macro + M_CP * ctx->total_nodes + d.fld_off;
+  double* out = ctx->d_diag + 2 * DIAG_MAX_WG;
+  HIP_OR_FAIL(ctx, launch_diagnostics(uy, cp, ctx->d_psi + d.fld_off, d.nx, d.ny, 0.5 / p.dy, ctx->d_diag, grid,
+                                      out, ctx->stream));
+  double host[2];
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(host, out, sizeof(host), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  if (umax) *umax = host[0];
+  if (current_sum) *current_sum = host[1];
+  return EKPNP_OK;
+}
+
+extern "C" {
+
 int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_steps, int* steps_out) {
+  const ApiRange range_("ekpnp_run_to_residual");
   if (!ctx) return EKPNP_ERR_INVALID;
   if (check_every < 2 || (check_every & 1)) return fail(ctx, EKPNP_ERR_INVALID, "check_every must be even and >= 2");
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
@@ -1409,6 +1471,7 @@ int ekpnp_run_to_residual(ekpnp_ctx* ctx, double tol, int check_every, int max_s
 }
 
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out) {
+  const ApiRange range_("ekpnp_autotune");
   if (!ctx || reps <= 0) return EKPNP_ERR_INVALID;
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -1523,6 +1586,7 @@ struct FileCloser {
 }  // namespace
 
 int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
+  const ApiRange range_("ekpnp_save_checkpoint");
   if (!ctx || !path) return EKPNP_ERR_INVALID;
   if (!ctx->has_fields) return fail(ctx, EKPNP_ERR_STATE, "nothing to save: no fields");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
@@ -1572,6 +1636,7 @@ int ekpnp_save_checkpoint(ekpnp_ctx* ctx, const char* path) {
 }
 
 int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
+  const ApiRange range_("ekpnp_load_checkpoint");
   if (!ctx || !path) return EKPNP_ERR_INVALID;
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -1639,6 +1704,7 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path) {
 }
 
 int ekpnp_copy_bandwidth(int device, size_t bytes, int reps, int grid, double* gbps_out, double* ms_out) {
+  const ApiRange range_("ekpnp_copy_bandwidth");
   if (!gbps_out || reps < 1 || grid < 1 || bytes < 16 || bytes % 16 != 0) return EKPNP_ERR_INVALID;
   if (hipSetDevice(device) != hipSuccess) return EKPNP_ERR_HIP;
   void* a = nullptr;
@@ -1681,6 +1747,7 @@ long long ekpnp_steps_done(const ekpnp_ctx* ctx) { return ctx ? ctx->steps : 0; 
 size_t ekpnp_device_bytes(const ekpnp_ctx* ctx) { return ctx ? ctx->dev_bytes : 0; }
 
 int ekpnp_time_stages(ekpnp_ctx* ctx, int nsteps, double* ms_out) {
+  const ApiRange range_("ekpnp_time_stages");
   if (!ctx || !ms_out || nsteps <= 0) return EKPNP_ERR_INVALID;
   if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "no populations");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));

@@ -172,6 +172,16 @@ hipError_t launch_psi_wall_rows(const StepBuffers& b, int chan, int nx, hipStrea
 constexpr long long NOALLOC_SCRATCH_DOUBLES = EKPNP_NOALLOC_SCRATCH_DOUBLES;  // 8 MiB
 hipError_t noalloc_scratch_init(int device);
 double* noalloc_scratch(int device);
+// Run-time diagnostics of one channel (umax and the top-row current sum, see
+// k_diag_partial): grid (1..DIAG_MAX_WG) work-groups of 256 lanes write
+// 2 x grid partials, then out[0] = umax, out[1] = sum over the top row of
+// c+ E_y (uy, cp, psi: the channel's planes, nx * ny each)
+constexpr int DIAG_MAX_WG = 1024;
+hipError_t launch_diagnostics(const double* uy, const double* cp, const double* psi, int nx, int ny,
+                              double half_inv_dy, double* partials, int grid, double* out, hipStream_t s);
+// umax and the top-row current sum of one channel on the device (the
+// context's stream, synchronised); either output may be null
+int channel_diagnostics(ekpnp_ctx* ctx, int channel, double* umax, double* current_sum);
 // dst = src, n_float4 16-byte values, non-temporal, grid work-groups of 256 lanes
 hipError_t launch_copy_nt(const void* src, void* dst, long long n_float4, int grid, hipStream_t s);
 

@@ -8,8 +8,10 @@
 //   charge_data      save_data_dmd    LBM.cu:1314-1348 (one c+ per line)
 //   current          current          LBM.cu:1394-1417
 //   umax.dat         record_umax      LBM.cu:1419-1451
-// All of them work on host copies of the channel's fields; the wall-row
-// extrapolations they apply change the copies only, never the device state.
+// The file writers work on host copies of the channel's fields; the
+// wall-row extrapolations they apply change the copies only, never the
+// device state.  current and record_umax reduce on the device and copy two
+// numbers back.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +19,7 @@
 #include <vector>
 
 #include "../../include/ekpnp.h"
+#include "ekpnp_internal.h"
 
 namespace {
 
@@ -186,25 +189,23 @@ int ekpnp_save_charge_data(ekpnp_ctx* ctx, int channel, const char* path) {
 
 int ekpnp_current(ekpnp_ctx* ctx, int channel, double* out) {
   if (!ctx || !out) return EKPNP_ERR_INVALID;
-  Fields f;
-  int rc = fetch(ctx, channel, f);
+  ekpnp_channel_params p;
+  int rc = ekpnp_get_params(ctx, channel, &p);
   if (rc) return rc;
-  extrap_top(f.cp, f);
-  double I = 0.0;
-  for (int x = 0; x < f.nx; ++x) I += f.cp[f.at(x, f.ny - 1)] * f.ey[f.at(x, f.ny - 1)];
-  *out = I * f.p.mob_pos * f.p.dy;
+  double sum = 0.0;
+  rc = ekpnp::channel_diagnostics(ctx, channel, nullptr, &sum);
+  if (rc) return rc;
+  *out = sum * p.mob_pos * p.dy;
   return EKPNP_OK;
 }
 
 int ekpnp_record_umax(ekpnp_ctx* ctx, int channel, double time, const char* path, double* umax_out) {
   if (!ctx) return EKPNP_ERR_INVALID;
-  Fields f;
-  int rc = fetch(ctx, channel, f);
-  if (rc) return rc;
-  extrap_top(f.uy, f);
-  // the reference's umax is the largest u_y (starting from 0)
+  // the reference's umax is the largest u_y (starting from 0), top row
+  // extrapolated
   double umax = 0.0;
-  for (double v : f.uy) umax = std::fmax(umax, v);
+  int rc = ekpnp::channel_diagnostics(ctx, channel, &umax, nullptr);
+  if (rc) return rc;
   if (umax_out) *umax_out = umax;
   if (path) {
     FILE* fp = std::fopen(path, "a");

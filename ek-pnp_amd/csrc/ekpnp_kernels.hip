@@ -1287,21 +1287,32 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const in
       spec[spec_idx(cd->ny, y0 + r, k)] = cadd(E, cmul(tw_half(twq, k, quarter), O));
     }
   } else {
-    // direct real DFT of one row
+    // direct real DFT of the tile's rows: their right-hand sides in LDS
+    // (row stride ldv), then one output (row, mode) per lane in the order of
+    // the spectrum layout, 8 modes of a row then the next row, so that a
+    // wave's stores are contiguous
+    const int ldv = nx | 1;
     double* v = reinterpret_cast<double*>(lds);
-    for (int n = threadIdx.x; n < nx; n += FFT_WG_ROWS) v[n] = rhs_at(re, Wl, ws, cd, n, y0);
+    for (int e = threadIdx.x; e < rows * nx; e += FFT_WG_ROWS) {
+      const int r = e / nx, n = e - r * nx;
+      v[r * ldv + n] = rhs_at(re, Wl, ws, cd, n, y0 + r);
+    }
     __syncthreads();
-    for (int k = threadIdx.x; k < nk; k += FFT_WG_ROWS) {
+    const int nblk = (nk + 7) >> 3;
+    for (int e = threadIdx.x; e < nblk * rows * 8; e += FFT_WG_ROWS) {
+      const int kb = e / (rows * 8), rem = e - kb * rows * 8, r = rem >> 3, k = kb * 8 + (rem & 7);
+      if (k >= nk) continue;
+      const double* __restrict__ vr = v + r * ldv;
       double2 acc = make_double2(0.0, 0.0);
       int idx = 0;
       for (int n = 0; n < nx; ++n) {
         const double2 w = tw[idx];
-        acc.x += v[n] * w.x;
-        acc.y += v[n] * w.y;
+        acc.x += vr[n] * w.x;
+        acc.y += vr[n] * w.y;
         idx += k;
         if (idx >= nx) idx -= nx;
       }
-      spec[spec_idx(cd->ny, y0, k)] = acc;
+      spec[spec_idx(cd->ny, y0 + r, k)] = acc;
     }
   }
 }
@@ -1560,21 +1571,29 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv(StepBuffers b, const in
     const long long ws = b.walls_stride;
     if (y0 == 1) copy_row_batched<FFT_WG_ROWS>(ps, Wl + W_PSI_LO * ws, nx);
     if (y0 + rows - 1 == ny - 2) copy_row_batched<FFT_WG_ROWS>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
-    for (int k = threadIdx.x; k < nk; k += FFT_WG_ROWS) lds[k] = spec[spec_idx(ny, y0, k)];
+    // the tile's spectra in LDS, [row][mode] (loads in the order of the
+    // spectrum layout), then one output node per lane, x fastest
+    const int nblk = (nk + 7) >> 3;
+    for (int e = threadIdx.x; e < nblk * rows * 8; e += FFT_WG_ROWS) {
+      const int kb = e / (rows * 8), rem = e - kb * rows * 8, r = rem >> 3, k = kb * 8 + (rem & 7);
+      if (k < nk) lds[r * nk + k] = spec[spec_idx(ny, y0 + r, k)];
+    }
     __syncthreads();
     const int kmax = (nx - 1) / 2;  // modes counted twice
     const bool nyq = (nx & 1) == 0;
-    for (int n = threadIdx.x; n < nx; n += FFT_WG_ROWS) {
-      double acc = lds[0].x;
+    for (int e = threadIdx.x; e < rows * nx; e += FFT_WG_ROWS) {
+      const int r = e / nx, n = e - r * nx;
+      const double2* __restrict__ z = lds + r * nk;
+      double acc = z[0].x;
       int idx = 0;
       for (int k = 1; k <= kmax; ++k) {
         idx += n;
         if (idx >= nx) idx -= nx;
         const double2 w = tw[idx];
-        acc += 2.0 * (lds[k].x * w.x + lds[k].y * w.y);
+        acc += 2.0 * (z[k].x * w.x + z[k].y * w.y);
       }
-      if (nyq) acc += (n & 1) ? -lds[nx / 2].x : lds[nx / 2].x;
-      ps[(long long)y0 * nx + n] = acc / (double)nx;
+      if (nyq) acc += (n & 1) ? -z[nx / 2].x : z[nx / 2].x;
+      ps[(long long)(y0 + r) * nx + n] = acc / (double)nx;
     }
   }
 }
@@ -2442,6 +2461,76 @@ hipError_t launch_sor_sweep(const StepBuffers& b, int chan, int nx, int ny, int 
 
 hipError_t launch_max_final(const double* v, int n, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, v, n, out);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Run-time diagnostics of one channel from its macroscopic planes and psi:
+//  * umax: the largest u_y over the channel, starting from 0, with the top
+//    row taken as 2 u_y(ny-2) - u_y(ny-3) (record_umax, LBM.cu:1419-1451);
+//  * the current's sum over the top row of c+ E_y, c+ extrapolated the same
+//    way and E_y(ny-1) = E_y(ny-2) = (psi(ny-3) - psi(ny-1)) / (2 dy)
+//    (current, LBM.cu:1394-1417; gpu_efield / gpu_bc, poisson.cu:145-174).
+// k_diag_partial: partials[2 wg] = the largest u_y of the work-group's nodes
+// (grid-stride), partials[2 wg + 1] = its share of the top-row sum;
+// k_diag_final combines them in a fixed order into out[0] = umax,
+// out[1] = the sum.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_diag_partial(const double* __restrict__ uy, const double* __restrict__ cp,
+                                                      const double* __restrict__ psi, int nx, int ny,
+                                                      double half_inv_dy, double* __restrict__ partials) {
+  __shared__ double wm[4], ws[4];
+  const long long nn = (long long)nx * ny, top = (long long)(ny - 1) * nx, nx2 = 2LL * nx;
+  double m = 0.0, sum = 0.0;
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < nn; i += (long long)gridDim.x * 256) {
+    if (i < top) {
+      m = fmax(m, uy[i]);
+    } else {
+      m = fmax(m, 2.0 * uy[i - nx] - uy[i - nx2]);
+      const double c = 2.0 * cp[i - nx] - cp[i - nx2];
+      const double ey = (psi[i - nx2] - psi[i]) * half_inv_dy;
+      sum += c * ey;
+    }
+  }
+  m = wave_max(m);
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0) {
+    wm[threadIdx.x >> 6] = m;
+    ws[threadIdx.x >> 6] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    partials[2 * blockIdx.x + 1] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_diag_final(const double* __restrict__ partials, int n,
+                                                    double* __restrict__ out) {
+  __shared__ double wm[4], ws[4];
+  double m = 0.0, sum = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) {
+    m = fmax(m, partials[2 * i]);
+    sum += partials[2 * i + 1];
+  }
+  m = wave_max(m);
+  sum = wave_sum(sum);
+  if ((threadIdx.x & 63) == 0) {
+    wm[threadIdx.x >> 6] = m;
+    ws[threadIdx.x >> 6] = sum;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    out[0] = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+    out[1] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+  }
+}
+
+hipError_t launch_diagnostics(const double* uy, const double* cp, const double* psi, int nx, int ny,
+                              double half_inv_dy, double* partials, int grid, double* out, hipStream_t s) {
+  if (ny < 3 || grid < 1 || grid > DIAG_MAX_WG) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_diag_partial, dim3(grid), dim3(256), 0, s, uy, cp, psi, nx, ny, half_inv_dy, partials);
+  hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(256), 0, s, partials, grid, out);
   return hipGetLastError();
 }
 

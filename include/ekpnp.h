@@ -310,10 +310,11 @@ int ekpnp_read_data(ekpnp_ctx* ctx, int channel, const char* path, double* time_
 int ekpnp_save_tecplot(ekpnp_ctx* ctx, int channel, double time, const char* path, int first);
 /* Appends c+ (top row extrapolated), one "%10.6f " value per line. */
 int ekpnp_save_charge_data(ekpnp_ctx* ctx, int channel, const char* path);
-/* Wall current K dy sum_x c+(x, ny-1) Ey(x, ny-1), c+ extrapolated. */
+/* Wall current K dy sum_x c+(x, ny-1) Ey(x, ny-1), c+ extrapolated; reduced
+ * on the device (FP64), two numbers copied back. */
 int ekpnp_current(ekpnp_ctx* ctx, int channel, double* out);
-/* max(0, max u_y) with the top row extrapolated; appended to path as
- * "%10.6f %10.6f\n" (time, umax) unless path is NULL. */
+/* max(0, max u_y) with the top row extrapolated (reduced on the device);
+ * appended to path as "%10.6f %10.6f\n" (time, umax) unless path is NULL. */
 int ekpnp_record_umax(ekpnp_ctx* ctx, int channel, double time, const char* path, double* umax_out);
 /* Exact restart: every channel's stored populations (in the storage
  * precision), storage references, psi, macroscopic fields, wall profiles,

@@ -102,7 +102,10 @@ def test_current_umax_and_charge_data(tmp_path):
         uy = s.get_field(ch, "uy")
         top = 2.0 * cp[-2] - cp[-3]
         want_i = np.sum(top * ey[-1]) * p.mob_pos * p.dy
-        assert s.current(ch) == pytest.approx(want_i, rel=1e-12, abs=1e-300)
+        # the device adds the terms in another order: a bound on the
+        # rounding of a sum of nx terms, relative to the sum of their sizes
+        scale = np.sum(np.abs(top * ey[-1])) * p.mob_pos * p.dy
+        assert abs(s.current(ch) - want_i) <= 1e-13 * scale, (s.current(ch), want_i, scale)
         uy2 = uy.copy()
         uy2[-1] = 2.0 * uy[-2] - uy[-3]
         want_u = max(0.0, uy2.max())
@@ -117,6 +120,36 @@ def test_current_umax_and_charge_data(tmp_path):
         cp2 = cp.copy()
         cp2[-1] = top
         assert np.abs(got - cp2.ravel()).max() <= 5.0000001e-7
+
+
+def test_current_and_umax_reduce_on_the_device_in_a_batch():
+    # larger channels than above, behind another channel in the context: the
+    # device reductions run over many work-groups and the top row spans
+    # several of them; same expectations as the reference's host loops
+    ps = [ekpnp.lattice_params(32, 24, tau=0.8, zeta=-1.5, kappa_h=4.0, e_x=2e-4),
+          ekpnp.lattice_params(512, 260, tau=0.7, zeta=-2.0, kappa_h=8.0, e_x=3e-4),
+          ekpnp.lattice_params(1024, 66, tau=0.9, zeta=-1.0, kappa_h=5.0, e_x=1e-4)]
+    s = ekpnp.Solver(ps, precision="fp32")
+    z = -2.0 + 0.5 * np.sin(2 * np.pi * np.arange(512) / 512)
+    s.set_wall_profile(1, psi_lo=z, psi_hi=z, cpos_lo=np.exp(-z), cpos_hi=np.exp(-z), cneg_lo=np.exp(z),
+                       cneg_hi=np.exp(z))
+    s.initialization(ekpnp.COLD_NOISE, 5)
+    s.init_equilibrium()
+    s.step(150)
+    for ch, p in enumerate(ps):
+        print("case channel", ch, p.nx, "x", p.ny, flush=True)
+        cp = s.get_field(ch, "cpos")
+        ey = s.get_field(ch, "ey")
+        uy = s.get_field(ch, "uy")
+        top = 2.0 * cp[-2] - cp[-3]
+        want_i = np.sum(top * ey[-1]) * p.mob_pos * p.dy
+        scale = np.sum(np.abs(top * ey[-1])) * p.mob_pos * p.dy
+        got_i = s.current(ch)
+        assert abs(got_i - want_i) <= 1e-13 * scale, (ch, got_i, want_i, scale)
+        uy2 = uy.copy()
+        uy2[-1] = 2.0 * uy[-2] - uy[-3]
+        assert s.record_umax(ch, 0.0, None) == max(0.0, uy2.max())
+    s.close()
 
 
 def test_tecplot_zone(tmp_path):
