@@ -21,6 +21,7 @@ import ctypes
 from typing import Sequence
 
 import numpy as np
+from scipy.linalg import solve_banded
 
 from . import ChannelParams, EkpnpError, lib
 
@@ -255,20 +256,13 @@ def pb_profile(p: ChannelParams, tol: float = 1e-13, max_iter: int = 200) -> np.
 
 
 def _thomas(lo, diag, up, rhs):
-    n = diag.size
-    c = np.empty(n)
-    d = np.empty(n)
-    c[0] = up[0] / diag[0]
-    d[0] = rhs[0] / diag[0]
-    for i in range(1, n):
-        m = diag[i] - lo[i] * c[i - 1]
-        c[i] = up[i] / m
-        d[i] = (rhs[i] - lo[i] * d[i - 1]) / m
-    x = np.empty(n)
-    x[-1] = d[-1]
-    for i in range(n - 2, -1, -1):
-        x[i] = d[i] - c[i] * x[i + 1]
-    return x
+    """Tridiagonal solve (LAPACK's banded solver): lo[i] x[i-1] + diag[i] x[i]
+    + up[i] x[i+1] = rhs[i]; lo[0] and up[-1] are not used."""
+    ab = np.zeros((3, diag.size))
+    ab[0, 1:] = up[:-1]
+    ab[1] = diag
+    ab[2, :-1] = lo[1:]
+    return solve_banded((1, 1), ab, rhs)
 
 
 def baseline(p: ChannelParams):

@@ -47,30 +47,37 @@ def xavg(s, ch, name):
     return s.get_field(ch, name).mean(axis=1)
 
 
-def run_to_residual(s, tol, every, max_steps, tag, chunk=100_000):
+def run_to_residual(s, tol, every, max_steps, tag, chunk=100_000, on_chunk=None, deadline=None):
     """run_to_residual in slices of `chunk` steps with a progress line per
     slice; a channel's count is the step at which it first met tol (channels
     that met it are stepped again briefly in later slices, until the first
-    check re-freezes them)."""
+    check re-freezes them).  on_chunk(indices) sees the channels that met tol
+    in each slice (their fields are steady when it runs); past `deadline`
+    (time.perf_counter()) no further slice starts."""
     n = len(s.channels)
     done = np.full(n, -1, dtype=np.int64)
     start = s.steps_done
     while (done < 0).any() and s.steps_done - start < max_steps:
+        if deadline is not None and time.perf_counter() > deadline:
+            log(f"{tag}: deadline reached")
+            break
         off = s.steps_done - start
         res = s.run_to_residual(tol, every, min(chunk, max_steps - off))
         new = (done < 0) & (res >= 0)
         done[new] = off + res[new]
         log(f"{tag}: {int((done >= 0).sum())}/{n} at residual {tol:g} after {s.steps_done - start} steps")
+        if on_chunk is not None and new.any():
+            on_chunk(np.flatnonzero(new))
     return done
 
 
-def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold"):
+def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold", chunk=100_000):
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.synchronize()
     t0 = time.perf_counter()
-    steps = run_to_residual(s, tol, every, max_steps, tag)
+    steps = run_to_residual(s, tol, every, max_steps, tag, chunk)
     s.synchronize()
     return s, steps, time.perf_counter() - t0
 
@@ -158,13 +165,26 @@ def predictor(net, residual):
     return run
 
 
-def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None, settle=100):
+def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None, settle=100,
+             chunk=100_000):
     """Warm arm.  Acceptance test: after `settle` steps (the local transient
     of starting from equilibrium populations) the u_x residual over the next
     `probe` steps must be <= accept and finite; a rejected channel restarts
     from its cold fields.  Step counts include settle + probe.
     corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
     t0 = time.perf_counter()
+    s, res, rejected = warm_start(channels, predict, precision, seed, probe, accept, corrupt, settle)
+    steps = run_to_residual(s, tol, every, max_steps, "warm", chunk)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    total = np.where(steps >= 0, steps + settle + probe, -1)
+    return s, total, wall, res, rejected
+
+
+def warm_start(channels, predict, precision, seed, probe, accept, corrupt=None, settle=100):
+    """A context started from predicted profiles, after `settle` + `probe`
+    steps and the acceptance test (rejected channels restarted cold):
+    (solver, probe residuals, rejected channel indices)."""
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
     cold = [{f: s.get_field(c, f) for f in ("rho", "ux", "uy", "cpos", "cneg")} for c in range(len(channels))]
@@ -190,11 +210,7 @@ def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, a
         s.set_fields(c, f["rho"], f["ux"], f["uy"], f["cpos"], f["cneg"])
     if rejected:
         s.init_equilibrium()
-    steps = run_to_residual(s, tol, every, max_steps, "warm")
-    s.synchronize()
-    wall = time.perf_counter() - t0
-    total = np.where(steps >= 0, steps + settle + probe, -1)
-    return s, total, wall, res, rejected
+    return s, res, rejected
 
 
 def summarize(cold, warm):
