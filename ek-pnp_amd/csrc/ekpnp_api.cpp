@@ -164,6 +164,9 @@ std::string validate(const ekpnp_channel_params& p, int ch) {
   if (!(p.magic_f > 0 && p.magic_pos > 0 && p.magic_neg > 0)) return bad("non-positive TRT magic parameter");
   if (!(p.rho0 > 0)) return bad("non-positive rho0");
   if (!(p.lx > 0)) return bad("non-positive lx");
+  // the x length sets the x-FFT wavenumbers, dx the lattice: LX = NX dx
+  // (LBM.h:35-42)
+  if (std::fabs(p.lx - p.nx * p.dx) > 1e-9 * p.lx) return bad("lx differs from nx * dx");
   if (!std::isfinite(p.poisson_coef) || !std::isfinite(p.force_coef)) return bad("non-finite coefficient");
   return std::string();
 }

@@ -148,7 +148,7 @@ typedef struct ekpnp_channel_params {
   double e_applied;    /* Ext, LBM.h:62 */
   double exf;          /* LBM.h:47 */
   double uw;           /* top-wall velocity, LBM.h:46 */
-  double lx;           /* Lx, the x period used by kx (main.cu:120-128)     */
+  double lx;           /* Lx = nx dx, the x period used by kx (main.cu:120-128); checked */
   double init_noise;   /* relative amplitude of the cold-start noise        */
   double inv_thermal_voltage; /* e/(kB T) in units of 1/psi: the Boltzmann
                           factor exp(-+ inv_vt psi) of the PB start (LBM.cu:143-144) */

@@ -90,6 +90,12 @@ def test_create_rejects_bad_parameters_before_touching_a_device():
     arr = (ekpnp.ChannelParams * 1)(p)
     assert lib.ekpnp_create(arr, 1, None, ctypes.byref(ctx)) == ekpnp.ERR_INVALID
     assert b"viscosity" in lib.ekpnp_create_error()
+    # a width changed without the x length (the x-FFT wavenumbers follow lx)
+    q = ekpnp.lattice_params(256, 16)
+    q.nx = 8
+    arr = (ekpnp.ChannelParams * 1)(q)
+    assert lib.ekpnp_create(arr, 1, None, ctypes.byref(ctx)) == ekpnp.ERR_INVALID
+    assert b"lx differs from nx * dx" in lib.ekpnp_create_error()
     big = ekpnp.lattice_params(16384, 16)
     arr = (ekpnp.ChannelParams * 1)(big)
     assert lib.ekpnp_create(arr, 1, None, ctypes.byref(ctx)) == ekpnp.ERR_UNSUPPORTED
