@@ -13,6 +13,8 @@ trained on every band:
   gen   --band B [--net N]   steady states of band B (cold without --net)
   train --bands ...          a network on the saved bands -> --out
   heldout --net N            the held-out comparison (one JSON line)
+  sweep --net N              the 256-channel batched sweep to the residual,
+                             warm-started (one JSON line)
 
 Training channels and held-out channels are x-uniform, so training
 channels run at n_x = 4 and held-out channels at n_x = 8 (the steady state
@@ -86,7 +88,8 @@ def gen(args):
         Y[c] = y
     log(f"band {args.band}: {int(ok.sum())}/{len(ps)} converged, median steps "
         f"{np.median(steps[ok]) if ok.any() else float('nan'):.0f}, wall {time.perf_counter() - t0:.1f} s")
-    np.savez(args.out, X=X, Y=Y, steps=steps, ny=np.array([p.ny for p in ps]), rejected=np.array(rejected, np.int64),
+    # targets in FP32 (the network computes in BF16)
+    np.savez(args.out, X=X, Y=Y.astype(np.float32), steps=steps, ny=np.array([p.ny for p in ps]), rejected=np.array(rejected, np.int64),
              band=args.band, warm=bool(args.net))
 
 
@@ -118,7 +121,8 @@ def heldout(args):
         ps = []
         for q in drawn:
             c = q.copy()
-            c.nx, c.lx = 8, 8.0
+            c.nx = 8
+            c.lx = c.nx * c.dx  # the x length sets the x-FFT wavenumbers
             ps.append(c)
     log(f"held out: {len(ps)} channels, n_x {args.nx}, n_y {min(p.ny for p in ps)}..{max(p.ny for p in ps)}")
     net = S.load_net(args.net)
@@ -128,20 +132,67 @@ def heldout(args):
     net.close()
     log(f"warm: median {np.median(warm[warm > 0]):.0f} steps, makespan {warm.max()}, wall {warm_wall:.1f} s, "
         f"rejected {len(rejected)}")
-    s, cold, cold_wall = T.run_cold(ps, "fp32", args.seed, TOL, EVERY, args.max_steps, chunk=CHUNK)
+    # the cold arm runs until every channel converged or the time limit: a
+    # channel still running then has a lower bound on its cold step count
+    deadline = time.perf_counter() + args.cold_time_limit if args.cold_time_limit > 0 else None
+    s, cold, cold_wall = T.run_cold(ps, "fp32", args.seed, TOL, EVERY, args.max_steps, chunk=CHUNK,
+                                    deadline=deadline)
+    cold_run = s.steps_done
     s.close()
-    log(f"cold: median {np.median(cold[cold > 0]):.0f} steps, makespan {cold.max()}, wall {cold_wall:.1f} s")
+    censored = [c for c in range(len(ps)) if cold[c] < 0]
+    log(f"cold: {len(ps) - len(censored)} converged (median {np.median(cold[cold > 0]):.0f} steps), "
+        f"{len(censored)} still running after {cold_run} steps, wall {cold_wall:.1f} s")
     summ = T.summarize(cold, warm)
+    ok = (cold > 0) & (warm > 0)
+    lb = np.where(ok, cold, cold_run) / np.maximum(warm, 1)  # per-channel gain, a lower bound where censored
+    ny = np.array([p.ny for p in ps])
+    by_ny = {}
+    for v in sorted(set(ny.tolist())):
+        m = ny == v
+        by_ny[int(v)] = {"channels": int(m.sum()), "cold_converged": int((m & ok).sum()),
+                         "warm_median": float(np.median(warm[m])),
+                         "gain_median_or_lower_bound": round(float(np.median(lb[m])), 2)}
     out = {"experiment": "time-to-steady at the surrogate configuration's scale, surrogate-warm vs cold",
            "tol": TOL, "check_every": EVERY, "precision": "fp32", "network": os.path.basename(args.net),
            "network_meta": S.net_meta(args.net), "heldout": {
                "channels": len(ps), "seed": args.seed, "redrawn": rej, "nx": args.nx,
                "ny": [int(p.ny) for p in ps], "nx_drawn": [int(p.nx) for p in drawn],
                "cold_steps": cold.tolist(), "warm_steps": warm.tolist(), "rejected": rejected,
+               "cold_censored": censored, "cold_steps_run": int(cold_run),
                "probe_residuals": [float(v) for v in res],
                "cold_wall_s": round(cold_wall, 2), "warm_wall_s": round(warm_wall, 2),
-               "wall_ratio": round(cold_wall / warm_wall, 3), "summary": summ}}
+               "summary_both_converged": summ,
+               "gain_median_all_lower_bound": round(float(np.median(lb)), 2),
+               "makespan_gain_lower_bound": round(float(max(cold_run, cold.max()) / warm.max()), 2),
+               "by_ny": by_ny}}
     print(json.dumps(out), flush=True)
+
+
+def sweep(args):
+    """The batched sweep (256 channels of seed 2: n_x 32..256, n_y 64..1024,
+    the bench's config c2) run to the residual from the network's warm
+    starts in one context; one JSON line."""
+    ps, rej = S.sample_channels(256, 2)
+    nodes = sum(p.nx * p.ny for p in ps)
+    log(f"sweep: {len(ps)} channels, {nodes} nodes, n_y {min(p.ny for p in ps)}..{max(p.ny for p in ps)}")
+    deadline = time.perf_counter() + args.time_limit if args.time_limit > 0 else None
+    net = S.load_net(args.net)
+    s, warm, wall, res, rejected = T.run_warm(ps, T.predictor(net, True), "fp32", 2, TOL, EVERY, args.max_steps,
+                                              200, 1e-4, chunk=CHUNK, deadline=deadline)
+    run = s.steps_done
+    red = s.reductions()
+    s.close()
+    net.close()
+    ok = warm > 0
+    log(f"sweep: {int(ok.sum())}/{len(ps)} at residual {TOL:g}, median {np.median(warm[ok]):.0f}, "
+        f"makespan {warm.max()}, wall {wall:.1f} s")
+    print(json.dumps({"experiment": "batched sweep (256 channels, seed 2) to the u_x residual, warm-started",
+                      "tol": TOL, "check_every": EVERY, "precision": "fp32", "network": os.path.basename(args.net),
+                      "channels": len(ps), "nodes": nodes, "converged": int(ok.sum()), "rejected": rejected,
+                      "steps": warm.tolist(), "steps_run": int(run), "median_steps": float(np.median(warm[ok])),
+                      "makespan": int(warm.max()), "wall_s": round(wall, 2),
+                      "nonfinite_channels": int((red[:, 6] > 0).sum()),
+                      "ny": [int(p.ny) for p in ps], "nx": [int(p.nx) for p in ps]}), flush=True)
 
 
 def main():
@@ -169,8 +220,14 @@ def main():
     h.add_argument("--seed", type=int, default=900000)
     h.add_argument("--nx", choices=("8", "drawn"), default="8")
     h.add_argument("--max-steps", type=int, default=20_000_000)
+    h.add_argument("--cold-time-limit", type=float, default=0.0,
+                   help="seconds for the cold arm; channels still running then are reported as censored")
+    w = sub.add_parser("sweep")
+    w.add_argument("--net", required=True)
+    w.add_argument("--max-steps", type=int, default=8_000_000)
+    w.add_argument("--time-limit", type=float, default=0.0)
     args = ap.parse_args()
-    {"gen": gen, "train": train, "heldout": heldout}[args.cmd](args)
+    {"gen": gen, "train": train, "heldout": heldout, "sweep": sweep}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -71,13 +71,13 @@ def run_to_residual(s, tol, every, max_steps, tag, chunk=100_000, on_chunk=None,
     return done
 
 
-def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold", chunk=100_000):
+def run_cold(channels, precision, seed, tol, every, max_steps, tag="cold", chunk=100_000, deadline=None):
     s = ekpnp.Solver(channels, precision=precision)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.synchronize()
     t0 = time.perf_counter()
-    steps = run_to_residual(s, tol, every, max_steps, tag, chunk)
+    steps = run_to_residual(s, tol, every, max_steps, tag, chunk, deadline=deadline)
     s.synchronize()
     return s, steps, time.perf_counter() - t0
 
@@ -166,7 +166,7 @@ def predictor(net, residual):
 
 
 def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, accept, corrupt=None, settle=100,
-             chunk=100_000):
+             chunk=100_000, deadline=None):
     """Warm arm.  Acceptance test: after `settle` steps (the local transient
     of starting from equilibrium populations) the u_x residual over the next
     `probe` steps must be <= accept and finite; a rejected channel restarts
@@ -174,7 +174,7 @@ def run_warm(channels, predict, precision, seed, tol, every, max_steps, probe, a
     corrupt(channel, psi, cp, cn, ux) may tamper with predictions."""
     t0 = time.perf_counter()
     s, res, rejected = warm_start(channels, predict, precision, seed, probe, accept, corrupt, settle)
-    steps = run_to_residual(s, tol, every, max_steps, "warm", chunk)
+    steps = run_to_residual(s, tol, every, max_steps, "warm", chunk, deadline=deadline)
     s.synchronize()
     wall = time.perf_counter() - t0
     total = np.where(steps >= 0, steps + settle + probe, -1)

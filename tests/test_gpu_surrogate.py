@@ -72,3 +72,32 @@ def test_corrupted_prediction_is_rejected_and_falls_back_to_cold(trained):
         assert warm[c] == cold[c] + 300, (c, warm[c], cold[c])
     for c in (0, 2, 3, 5):
         assert warm[c] < cold[c]
+
+
+NET_PATH = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "ek-pnp_amd", "ekpnp",
+                        "warmstart_net.npz")
+
+
+@pytest.mark.skipif(not os.path.exists(NET_PATH), reason="no committed warm-start network")
+@pytest.mark.timeout(600)
+def test_committed_network_meets_the_fivefold_target():
+    # the network committed with the library (trained on the batched
+    # solver's steady states over n_y 64..1024) on held-out channels of
+    # another seed and width, n_y 64 and 128 (cold runs of taller channels
+    # take millions of steps): every warm start kept, faster than cold on
+    # every channel, and a median per-channel gain of at least 5
+    net = S.load_net(NET_PATH)
+    ho, _ = S.sample_channels(16, 903000, nx_choices=(32,), ny_choices=(64, 128))
+    s, cold, _ = T.run_cold(ho, "fp32", 903000, 1e-8, 100, 3_000_000)
+    s.close()
+    s, warm, _, res, rejected = T.run_warm(ho, T.predictor(net, True), "fp32", 903000, 1e-8, 100, 3_000_000,
+                                           200, ACCEPT)
+    s.close()
+    net.close()
+    gain = cold / np.maximum(warm, 1)
+    print("cold", cold.tolist(), "warm", warm.tolist(), "rejected", rejected, "median gain",
+          float(np.median(gain)), flush=True)
+    assert (cold > 0).all() and (warm > 0).all()
+    assert not rejected
+    assert (warm < cold).all()
+    assert np.median(gain) >= 5.0
