@@ -308,7 +308,8 @@ void drop_graphs(ekpnp_ctx* c) {
 hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft_persist) {
   c->fft_tiles.clear();
   c->tri_tiles.clear();
-  unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;
+  unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;  // rows; rows + quadrant table
+  unsigned lds_pow2_t2 = 16;                                 // rows + two-level table
   int n_fft_pow2 = 0;
   for (int pass = 0; pass < 2; ++pass) {
     for (int i = 0; i < c->nchan; ++i) {
@@ -323,6 +324,8 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
         const int padded = (rows * N + 15) & ~15;
         lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(padded * sizeof(double2)));
         lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((padded + d.nx / 4) * sizeof(double2)));
+        const int t2 = 64 + std::max(1, d.nx / 64);  // the kernels' two-level twiddle table
+        lds_pow2_t2 = std::max<unsigned>(lds_pow2_t2, (unsigned)((padded + t2) * sizeof(double2)));
       } else {
         // direct-DFT tiles: as many whole rows as fit fft_elems values and
         // kDftLds bytes of LDS (a row of nx | 1 right-hand-side values, or
@@ -374,12 +377,13 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   c->sb.n_tri_tiles = (int)c->tri_tiles.size();
   // rows + quadrant twiddle table in LDS when the device allows that much
   // dynamic LDS per work-group and the table costs no resident work-group per
-  // CU (nx = 8192: 96 KB with the table, one work-group per CU, 64 KB
-  // without, two); otherwise twiddles are read from global memory (same values)
+  // CU (nx = 8192: 96 KB with the table, one work-group per CU); otherwise
+  // rows + the two-level table (nx = 8192: 67 KB, two work-groups per CU),
+  // whose twiddles are products of two quadrant-table values
   c->sb.fft_tw_lds = (lds_pow2_tw <= (unsigned)c->max_lds &&
-                      fft_one_tile_blocks_per_cu(true, lds_pow2_tw) >= fft_one_tile_blocks_per_cu(false, lds_pow2))
+                      fft_one_tile_blocks_per_cu(true, lds_pow2_tw) >= fft_one_tile_blocks_per_cu(false, lds_pow2_t2))
                          ? 1 : 0;
-  c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2;
+  c->sb.fft_lds_pow2 = c->sb.fft_tw_lds ? lds_pow2_tw : lds_pow2_t2;
   c->sb.fft_lds_dft = lds_dft;
   c->sb.tri_lmax = tri_lmax;
   // the persistent kernels take tiles of <= FFT_PERSIST_ELEMS values and keep

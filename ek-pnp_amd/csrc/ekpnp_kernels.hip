@@ -1036,6 +1036,40 @@ __device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t,
   return make_double2(neg_if(a, nre), neg_if(b, nim));
 }
 
+// Two-level twiddle table, for rows whose quadrant table (nx / 4 values)
+// would cost a resident work-group of LDS: F[f] = W^f for f < 64, then
+// C[c] = W^(64 c) for c < max(1, nx / 64), W = e^{-2 pi i / nx}, both taken
+// from the quadrant table; W^t = C[t >> 6] F[t & 63], one complex product
+// (exact where either factor is 1).  At most 192 values (3 KB) for nx <= 8192.
+constexpr int TW2_LGF = 6, TW2_FINE = 1 << TW2_LGF;
+__host__ __device__ constexpr int tw2_count(int nx) { return TW2_FINE + (nx / TW2_FINE > 1 ? nx / TW2_FINE : 1); }
+
+template <int WGT>
+__device__ __forceinline__ void tw2_fill(double2* t2, const double2* __restrict__ q, int nx, int lgnx) {
+  const int n = tw2_count(nx);
+  for (int i = threadIdx.x; i < n; i += WGT) {
+    const int t = i < TW2_FINE ? i : (i - TW2_FINE) << TW2_LGF;
+    t2[i] = tw_full<-1>(q, t & (nx - 1), lgnx);
+  }
+}
+
+// e^{SIGN 2 pi i t/nx}, 0 <= t < nx: quadrant table or two-level table
+template <int SIGN, bool TW2>
+__device__ __forceinline__ double2 tw_get(const double2* __restrict__ q, int t, int lgnx) {
+  if (TW2) {
+    const double2 w = cmul(q[TW2_FINE + (t >> TW2_LGF)], q[t & (TW2_FINE - 1)]);
+    return SIGN > 0 ? cconj(w) : w;
+  }
+  return tw_full<SIGN>(q, t, lgnx);
+}
+
+// e^{-2 pi i j/nx} for 0 <= j <= nx / 2 (exactly -1 at nx / 2)
+template <bool TW2>
+__device__ __forceinline__ double2 tw_half_get(const double2* __restrict__ q, int j, int quarter, int lgnx) {
+  if (TW2) return j == 2 * quarter ? make_double2(-1.0, 0.0) : tw_get<-1, true>(q, j, lgnx);
+  return tw_half(q, j, quarter);
+}
+
 // LDS position of row element i: bits 0-3 XOR bits 4-7.  The Stockham
 // passes write with strides 8 (first pass) and 8 x 16 (second) elements,
 // which without the swizzle put 8 (resp. 2) of every 16 lanes of a
@@ -1053,7 +1087,7 @@ __device__ __forceinline__ int sw(int i) { return i ^ ((i >> 4) & 15); }
 // one pass of 2^(lgN mod 4) when lgN is not a multiple of 4, then radix 16.
 // Each lane holds 16 values per pass.  twq is the quadrant table of e^{-2 pi i j/nx}.
 // ---------------------------------------------------------------------------
-template <int SIGN, int R, int WGT>
+template <int SIGN, int R, int WGT, bool TW2>
 __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int lgn, int lgs,
                                               const double2* __restrict__ twq, int lgnx) {
   constexpr int LGR = R == 16 ? 4 : (R == 8 ? 3 : (R == 4 ? 2 : 1));
@@ -1087,23 +1121,23 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
       // W^(p u): table values for u = 1, 2, 4, 8, products (at most three
       // roundings deep) for the rest
       const int t1 = p << (lgnx - lgn);
-      const double2 w1 = tw_full<SIGN>(twq, t1, lgnx);
+      const double2 w1 = tw_get<SIGN, TW2>(twq, t1, lgnx);
       if (R == 2) {
         x[sw(ob + s)] = cmul(v[c][1], w1);
       } else {
-        const double2 w2 = tw_full<SIGN>(twq, 2 * t1, lgnx);
+        const double2 w2 = tw_get<SIGN, TW2>(twq, 2 * t1, lgnx);
         x[sw(ob + s)] = cmul(v[c][1], w1);
         x[sw(ob + 2 * s)] = cmul(v[c][2], w2);
         const double2 w3 = cmul(w2, w1);
         x[sw(ob + 3 * s)] = cmul(v[c][3], w3);
         if (R >= 8) {
-          const double2 w4 = tw_full<SIGN>(twq, 4 * t1, lgnx);
+          const double2 w4 = tw_get<SIGN, TW2>(twq, 4 * t1, lgnx);
           x[sw(ob + 4 * s)] = cmul(v[c][4], w4);
           x[sw(ob + 5 * s)] = cmul(v[c][5], cmul(w4, w1));
           x[sw(ob + 6 * s)] = cmul(v[c][6], cmul(w4, w2));
           x[sw(ob + 7 * s)] = cmul(v[c][7], cmul(w4, w3));
           if (R == 16) {
-            const double2 w8 = tw_full<SIGN>(twq, 8 * t1, lgnx);
+            const double2 w8 = tw_get<SIGN, TW2>(twq, 8 * t1, lgnx);
             x[sw(ob + 8 * s)] = cmul(v[c][8], w8);
             x[sw(ob + 9 * s)] = cmul(v[c][9], cmul(w8, w1));
             x[sw(ob + 10 * s)] = cmul(v[c][10], cmul(w8, w2));
@@ -1123,26 +1157,26 @@ __device__ __forceinline__ void stockham_pass(double2* x, int rows, int lgN, int
 
 // WGT lanes; MAXR = 16 (16 values per lane per pass) or 8 (one more pass,
 // half the registers: the 512-lane one-tile kernels)
-template <int SIGN, int WGT, int MAXR>
+template <int SIGN, int WGT, int MAXR, bool TW2 = false>
 __device__ __forceinline__ void stockham(double2* x, int rows, int N, int lgN, const double2* __restrict__ twq, int lgnx) {
   constexpr int LGM = MAXR == 16 ? 4 : 3;
   int lgn = lgN, lgs = 0;
   const int r = lgN % LGM;
   if (r == 1) {
-    stockham_pass<SIGN, 2, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 2, WGT, TW2>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 1;
     lgs += 1;
   } else if (r == 2) {
-    stockham_pass<SIGN, 4, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 4, WGT, TW2>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 2;
     lgs += 2;
   } else if (r == 3) {
-    stockham_pass<SIGN, 8, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, 8, WGT, TW2>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= 3;
     lgs += 3;
   }
   while (lgn >= LGM) {
-    stockham_pass<SIGN, MAXR, WGT>(x, rows, lgN, lgn, lgs, twq, lgnx);
+    stockham_pass<SIGN, MAXR, WGT, TW2>(x, rows, lgN, lgn, lgs, twq, lgnx);
     lgn -= LGM;
     lgs += LGM;
   }
@@ -1232,7 +1266,10 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const in
     const int N = nx >> 1, lgN = ilog2(N), lgnx = lgN + 1, quarter = nx >> 2;
     const int total = rows << lgN;  // <= FFT_MAX_ELEMS
     const double pc = cd->pc_dy2;
-    double2* twl = lds + ((total + 15) & ~15);  // quadrant twiddles after the (swizzled) rows
+    // twiddles after the (swizzled) rows: the quadrant table, or without
+    // it (TWL false) the two-level table, filled once the rows' loads are
+    // in flight
+    double2* twl = lds + ((total + 15) & ~15);
     if (TWL) lds_dma_copy<FFT_WG_ROWS>(twl, tw, quarter);
     // the tile's rows are contiguous in rho_e: element e = (r, n) holds
     // x[2n], x[2n+1] of row y0 + r as one 16-byte value (rows start 16-byte aligned)
@@ -1259,7 +1296,8 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const in
         }
       }
     }
-    const double2* twq = TWL ? twl : tw;
+    if (!TWL) tw2_fill<FFT_WG_ROWS>(twl, tw, nx, lgnx);
+    const double2* twq = twl;
 #pragma unroll
     for (int c = 0; c < FFT_ROW_SLOTS; ++c) {
       const int e = threadIdx.x + c * FFT_WG_ROWS;
@@ -1273,7 +1311,7 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const in
       }
     }
     __syncthreads();
-    stockham<-1, FFT_WG_ROWS, 8>(lds, rows, N, lgN, twq, lgnx);
+    stockham<-1, FFT_WG_ROWS, 8, !TWL>(lds, rows, N, lgN, twq, lgnx);
     // split the packed transform: X[k] = E[k] + W^k O[k]
     // rows vary fastest across lanes: 8 modes x `rows` rows are contiguous
     const float inv_rows = 1.0f / (float)rows;
@@ -1284,7 +1322,7 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_fwd(StepBuffers b, const in
       const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
       const double2 dd = csub(zk, zc);
       const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);  // (zk - zc)/(2i)
-      spec[spec_idx(cd->ny, y0 + r, k)] = cadd(E, cmul(tw_half(twq, k, quarter), O));
+      spec[spec_idx(cd->ny, y0 + r, k)] = cadd(E, cmul(tw_half_get<!TWL>(twq, k, quarter, lgnx), O));
     }
   } else {
     // direct real DFT of the tile's rows: their right-hand sides in LDS
@@ -1526,8 +1564,9 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv(StepBuffers b, const in
         xa[c] = spec[spec_idx(ny, y0 + r, p)];
         xb[c] = spec[spec_idx(ny, y0 + r, N - p)];
       }
-    if (TWL) __syncthreads();  // twiddles landed in LDS
-    const double2* twq = TWL ? twl : tw;
+    if (!TWL) tw2_fill<FFT_WG_ROWS>(twl, tw, nx, lgnx);  // the spectrum's loads are in flight
+    __syncthreads();  // twiddles in LDS
+    const double2* twq = twl;
     // z[k] = E + i O, E = (X[k] + conj X[N-k]) / 2,
     // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
 #pragma unroll
@@ -1540,20 +1579,20 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv(StepBuffers b, const in
           const double2 xk = xa[c], xc = cconj(xb[c]);
           const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
           const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
-                                 cconj(tw_half(twq, p, quarter)));
+                                 cconj(tw_half_get<!TWL>(twq, p, quarter, lgnx)));
           lds[sw(rb + p)] = make_double2(E.x - O.y, E.y + O.x);
         }
         if (p > 0 && p < half) {
           const double2 xk = xb[c], xc = cconj(xa[c]);
           const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
           const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
-                                 cconj(tw_half(twq, N - p, quarter)));
+                                 cconj(tw_half_get<!TWL>(twq, N - p, quarter, lgnx)));
           lds[sw(rb + N - p)] = make_double2(E.x - O.y, E.y + O.x);
         }
       }
     }
     __syncthreads();
-    stockham<+1, FFT_WG_ROWS, 8>(lds, rows, N, lgN, twq, lgnx);
+    stockham<+1, FFT_WG_ROWS, 8, !TWL>(lds, rows, N, lgN, twq, lgnx);
     const double invN = 1.0 / (double)N;
     double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)y0 * nx);
     for (int e = threadIdx.x; e < rows * N; e += FFT_WG_ROWS) {

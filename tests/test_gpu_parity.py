@@ -97,9 +97,13 @@ def test_stream_collide_save_then_fast_poisson_matches_step():
     assert_parity(s, 0, o, FP64_TOL, tag="split ops")
 
 
-@pytest.mark.parametrize("nx,ny", [(64, 130), (48, 66), (256, 1026), (32, 2050), (8192, 6)])
+@pytest.mark.parametrize("nx,ny", [(64, 130), (48, 66), (256, 1026), (32, 2050), (8192, 6),
+                                   (4, 300), (33, 70), (2047, 5), (2046, 9)])
 def test_poisson_only(nx, ny):
-    # psi from random c+- with per-x wall potentials, both oracle methods
+    # psi from random c+- with per-x wall potentials, both oracle methods;
+    # the last four widths take the direct DFT with several rows per tile
+    # (up to 298 rows of nx = 4, ragged last tiles under the smaller plans;
+    # odd widths; two rows of the widest non-power-of-two rows per tile)
     rng = np.random.default_rng(nx * 7919 + ny)
     p = ekpnp.lattice_params(nx, ny, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
     shp = (ny, nx)
