@@ -252,12 +252,14 @@ def test_fp32_arithmetic_variant_is_bounded():
     # FP32 collision arithmetic (compute_fp64=False) is an option for
     # throughput comparisons, not the default, and does NOT meet the 1e-5 bar:
     # its u_x error grows to ~6e-5 over 20000 steps (the small drift velocity
-    # is a difference of O(1) population sums; measured 6.0e-5), bounded here
+    # is a difference of O(1) population sums), bounded here a few times
+    # above what it measures at step 20000 (u_x 6.0e-5, psi 2.8e-7, c+-
+    # 3.4e-8), so that a change in its error shows
     errs = _fp32_errors_64x128(False)
     for f in ("cpos", "cneg"):
-        assert errs[f] < 1e-5, (f, errs[f])
-    assert errs["psi"] < 1e-4
-    assert errs["ux"] < 1e-3
+        assert errs[f] < 2e-7, (f, errs[f])
+    assert errs["psi"] < 3e-6
+    assert errs["ux"] < 2e-4
 
 
 @pytest.mark.parametrize("nx,ny", [(256, 40), (512, 24), (64, 37), (128, 20), (192, 9), (320, 5)])
