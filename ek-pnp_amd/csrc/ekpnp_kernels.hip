@@ -55,8 +55,13 @@ constexpr double W0 = 4.0 / 9.0, WS = 1.0 / 9.0, WD = 1.0 / 36.0;  // LBM.h:105-
 #ifndef EKPNP_NT_LOAD
 #define EKPNP_NT_LOAD 1
 #endif
+// k_lbm asks for 3 waves per SIMD: its FP32-storage, FP64-arithmetic
+// variants otherwise take 169-171 VGPRs (2 waves); at 168 the compiler spills
+// 1-4 VGPRs and the kernel is 5-14 % faster (4096x1024: 214 -> 203 us;
+// 5000 channels of nx = 4: 214 -> 184 us); variants already within 168 are
+// unchanged
 #ifndef EKPNP_LBM_WAVES
-#define EKPNP_LBM_WAVES 0
+#define EKPNP_LBM_WAVES 3
 #endif
 #if EKPNP_LBM_WAVES > 0
 #define LBM_WAVES_ATTR __attribute__((amdgpu_waves_per_eu(EKPNP_LBM_WAVES)))
