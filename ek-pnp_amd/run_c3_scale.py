@@ -103,7 +103,7 @@ def train(args):
             log(f"{path}: {int(ok.sum())} states")
     X, Y = np.concatenate(Xs), np.concatenate(Ys)
     t0 = time.perf_counter()
-    net, hist = T.train(X, Y, args.epochs, args.batch, args.lr, args.seed)
+    net, hist = T.train(X, Y, args.epochs, args.batch, args.lr, args.seed, n_hidden=args.hidden)
     log(f"trained on {len(X)} states in {time.perf_counter() - t0:.1f} s, val mse {hist[-1][2]:.3e}")
     ny = sorted({int(v) for path in args.bands for v in np.load(path)["ny"]})
     S.save_net(args.out, net, command=" ".join(["run_c3_scale.py"] + sys.argv[1:]), train_channels=len(X),
@@ -212,6 +212,7 @@ def main():
     t.add_argument("--epochs", type=int, default=3000)
     t.add_argument("--batch", type=int, default=256)
     t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--hidden", type=int, default=256, help="units per hidden layer (multiple of 32)")
     t.add_argument("--seed", type=int, default=1000)
     t.add_argument("--out", required=True)
     h = sub.add_parser("heldout")

@@ -126,13 +126,13 @@ def generate(channels, precision, seed, tol, every, max_steps, residual):
     return X, T, steps, wall
 
 
-def train(X, T, epochs, batch, lr0, seed):
+def train(X, T, epochs, batch, lr0, seed, n_hidden=256):
     rng = np.random.default_rng(seed)
     n = X.shape[0]
     nval = max(1, n // 10)
     perm = rng.permutation(n)
     val, tr = perm[:nval], perm[nval:]
-    net = S.MLP(max_batch=batch, seed=seed)
+    net = S.MLP(n_hidden=n_hidden, max_batch=batch, seed=seed)
     steps_per_epoch = max(1, len(tr) // batch)
     total = epochs * steps_per_epoch
     it = 0
