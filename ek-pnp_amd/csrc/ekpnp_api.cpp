@@ -172,7 +172,7 @@ std::string validate(const ekpnp_channel_params& p, int ch) {
 }
 
 // EKPNP_CHECK_* flags of one channel (ekpnp_check_params)
-uint32_t check_flags(const ekpnp_channel_params& p, double* kappa_dy, double* drift) {
+uint32_t check_flags(const ekpnp_channel_params& p, double* kappa_dy, double* drift, double* flow) {
   const double kappa = std::sqrt(std::max(0.0, 2.0 * p.poisson_coef * p.c_bulk * p.inv_thermal_voltage));
   const double zw = std::max(std::fabs(p.psi_lo), std::fabs(p.psi_hi));
   const double kmax = std::max(std::fabs(p.mob_pos), std::fabs(p.mob_neg));
@@ -180,9 +180,15 @@ uint32_t check_flags(const ekpnp_channel_params& p, double* kappa_dy, double* dr
   const double ew = ivt > 0.0 ? 2.0 * kappa / ivt * std::sinh(0.5 * ivt * zw) : 0.0;  // wall field
   *kappa_dy = kappa * p.dy;
   *drift = kmax * ew * p.dt / p.dx;
+  // the bulk electro-osmotic velocity: nu rho0 u'' = -force_coef rho_e E_x
+  // and psi'' = -poisson_coef rho_e give u = force_coef E_x (psi - zeta) /
+  // (poisson_coef nu rho0), in dx/dt units
+  const double den = p.poisson_coef * p.nu * p.rho0;
+  *flow = den != 0.0 ? std::fabs(p.force_coef * p.e_applied * zw / den) * p.dt / p.dx : 0.0;
   uint32_t f = 0;
   if (!(*kappa_dy <= 0.4)) f |= EKPNP_CHECK_DEBYE_UNRESOLVED;
   if (!(*drift <= 0.05)) f |= EKPNP_CHECK_WALL_DRIFT;
+  if (!(*flow <= 0.1)) f |= EKPNP_CHECK_FLOW_FAST;
   return f;
 }
 
@@ -677,8 +683,8 @@ int ekpnp_check_params(const ekpnp_channel_params* channels, int n_channels, uin
   if (!channels || n_channels <= 0) return EKPNP_ERR_INVALID;
   int flagged = 0;
   for (int i = 0; i < n_channels; ++i) {
-    double kdy = 0.0, drift = 0.0;
-    const uint32_t f = check_flags(channels[i], &kdy, &drift);
+    double kdy = 0.0, drift = 0.0, flow = 0.0;
+    const uint32_t f = check_flags(channels[i], &kdy, &drift, &flow);
     if (flags_out) flags_out[i] = f;
     flagged += f != 0;
   }
@@ -712,16 +718,17 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
       g_create_error = m;
       return (m.find("above") != std::string::npos) ? EKPNP_ERR_UNSUPPORTED : EKPNP_ERR_INVALID;
     }
-    double kdy = 0.0, drift = 0.0;
-    const uint32_t f = check_flags(channels[i], &kdy, &drift);
+    double kdy = 0.0, drift = 0.0, flow = 0.0;
+    const uint32_t f = check_flags(channels[i], &kdy, &drift, &flow);
     if (o.reject_unresolved && f) {
-      char buf[256];
-      snprintf(buf, sizeof buf, "channel %d: %s%s (kappa dy = %.3g, limit 0.4; wall drift = %.3g, limit 0.05)", i,
-               (f & EKPNP_CHECK_DEBYE_UNRESOLVED) ? "Debye length unresolved" : "",
-               (f & EKPNP_CHECK_WALL_DRIFT) ? ((f & EKPNP_CHECK_DEBYE_UNRESOLVED) ? ", wall drift too fast"
-                                                                                   : "wall drift too fast")
-                                            : "",
-               kdy, drift);
+      std::string why;
+      if (f & EKPNP_CHECK_DEBYE_UNRESOLVED) why += "Debye length unresolved";
+      if (f & EKPNP_CHECK_WALL_DRIFT) why += std::string(why.empty() ? "" : ", ") + "wall drift too fast";
+      if (f & EKPNP_CHECK_FLOW_FAST) why += std::string(why.empty() ? "" : ", ") + "electro-osmotic flow too fast";
+      char buf[320];
+      snprintf(buf, sizeof buf,
+               "channel %d: %s (kappa dy = %.3g, limit 0.4; wall drift = %.3g, limit 0.05; flow = %.3g, limit 0.1)", i,
+               why.c_str(), kdy, drift, flow);
       g_create_error = buf;
       return EKPNP_ERR_UNSTABLE;
     }

@@ -37,6 +37,11 @@ int main() {
   EXPECT(ekpnp_check_params(p, 0, flags) == EKPNP_ERR_INVALID);
   EXPECT(ekpnp_check_params(nullptr, 1, flags) == EKPNP_ERR_INVALID);
   EXPECT(ekpnp_check_params(p, 4, nullptr) == 2);
+  // tau 0.51 under a strong field: electro-osmotic flow |zeta| E / nu = 6
+  ekpnp_channel_params q;
+  EXPECT(ekpnp_lattice_params(&q, 64, 128, 0.51, -2.0, 8.0, 1e-2, 0.05) == EKPNP_OK);
+  uint32_t fq = 0;
+  EXPECT(ekpnp_check_params(&q, 1, &fq) == 1 && fq == EKPNP_CHECK_FLOW_FAST);
 
   ekpnp_options o;
   std::memset(&o, 0x5a, sizeof o);

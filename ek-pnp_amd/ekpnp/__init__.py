@@ -247,12 +247,13 @@ def copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10, grid: int = 1024, devi
     return {"GBps": g.value, "ms_per_copy": ms.value, "bytes": int(nbytes), "grid": int(grid), "reps": int(reps)}
 
 
-CHECK_DEBYE_UNRESOLVED, CHECK_WALL_DRIFT = 0x1, 0x2
+CHECK_DEBYE_UNRESOLVED, CHECK_WALL_DRIFT, CHECK_FLOW_FAST = 0x1, 0x2, 0x4
 
 
 def check_params(channels: Sequence[ChannelParams]) -> np.ndarray:
     """EKPNP_CHECK_* flags per channel (ekpnp_check_params; no device needed):
-    Debye length unresolved (kappa dy > 0.4), ions' wall drift above 0.05."""
+    Debye length unresolved (kappa dy > 0.4), ions' wall drift above 0.05,
+    electro-osmotic (Helmholtz-Smoluchowski) flow above 0.1 dx/dt."""
     if isinstance(channels, ChannelParams):
         channels = [channels]
     arr = (ChannelParams * len(channels))(*channels)
@@ -485,5 +486,5 @@ __all__ = [
     "ChannelParams", "Options", "Solver", "EkpnpError", "lib", "reference_params", "lattice_params",
     "FIELDS", "SPECIES", "RED_NAMES", "COLD_NOISE", "REFERENCE", "PB_PICARD", "POP_FP32", "POP_FP64",
     "FLAG_ION_DRIFT_APPLIED", "FLAG_G0_FLUID_U", "SIGNATURES", "LIB_PATH", "REBASE_INTERVAL", "rebase_steps", "copy_bandwidth", "check_params",
-    "CHECK_DEBYE_UNRESOLVED", "CHECK_WALL_DRIFT",
+    "CHECK_DEBYE_UNRESOLVED", "CHECK_WALL_DRIFT", "CHECK_FLOW_FAST",
 ]

@@ -178,11 +178,16 @@ typedef struct ekpnp_options {
 #define EKPNP_CHECK_DEBYE_UNRESOLVED 0x1u /* kappa dy > 0.4                      */
 #define EKPNP_CHECK_WALL_DRIFT 0x2u       /* Gouy-Chapman wall drift of the ions
                                              > 0.05 lattice speed (dx/dt)        */
+#define EKPNP_CHECK_FLOW_FAST 0x4u        /* Helmholtz-Smoluchowski flow of the
+                                             applied field > 0.1 lattice speed   */
 /* Resolution and stability rule for each channel, without a device:
  * kappa = sqrt(2 poisson_coef c_bulk inv_thermal_voltage) (linearised
  * Poisson-Boltzmann), the Debye length resolved (kappa dy <= 0.4) and the
  * ions' drift in the Gouy-Chapman wall field, max|K| (2 kappa / inv_vt)
- * sinh(inv_vt max|psi_wall| / 2), at most 0.05 dx/dt.  Channels outside the
+ * sinh(inv_vt max|psi_wall| / 2), at most 0.05 dx/dt; and the electro-
+ * osmotic flow the applied field drives (tau -> 1/2 or a huge E), the
+ * Helmholtz-Smoluchowski velocity |force_coef e_applied max|psi_wall| /
+ * (poisson_coef nu rho0)|, at most 0.1 dx/dt.  Channels outside the
  * rule need not fail, but strongly charged thin-layer channels past it have
  * gone non-finite (the batched sweep's sampler redraws them).  flags_out
  * (may be NULL, else n values) receives EKPNP_CHECK_* per channel; returns

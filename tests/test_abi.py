@@ -121,13 +121,19 @@ def test_check_params_flags_unresolved_and_fast_drift_channels():
     debye = ekpnp.lattice_params(64, 40, zeta=-0.5, kappa_h=9.0)          # kappa dy 0.46, drift 0.012
     drift = ekpnp.lattice_params(64, 200, zeta=-4.0, kappa_h=30.0)        # kappa dy 0.30, drift 0.11
     ref = ekpnp.reference_params()                                        # kappa dy 0.11 (SI units)
-    flags = ekpnp.check_params([ok, debye, drift, ref])
-    assert flags.tolist() == [0, ekpnp.CHECK_DEBYE_UNRESOLVED, ekpnp.CHECK_WALL_DRIFT, 0]
+    # tau -> 1/2 with a strong field: the Helmholtz-Smoluchowski velocity
+    # |zeta| E_x / nu (force_coef = poisson_coef, rho0 = 1) = 2e-2 / (0.01/3) = 6
+    flow = ekpnp.lattice_params(64, 128, tau=0.51, zeta=-2.0, kappa_h=8.0, e_x=1e-2)
+    # just inside the flow limit: 0.09 at tau 0.8
+    slow = ekpnp.lattice_params(64, 128, tau=0.8, zeta=-3.0, kappa_h=8.0, e_x=3e-3)
+    flags = ekpnp.check_params([ok, debye, drift, ref, flow, slow])
+    assert flags.tolist() == [0, ekpnp.CHECK_DEBYE_UNRESOLVED, ekpnp.CHECK_WALL_DRIFT, 0, ekpnp.CHECK_FLOW_FAST, 0]
     kappa = 30.0 / (0.5 * 199)
     assert 0.05 < 0.05 * 2 * kappa * math.sinh(2.0) and kappa < 0.4
+    assert abs(2.0 * 1e-2 / ((0.51 - 0.5) / 3) - 6.0) < 1e-9 and 3.0 * 3e-3 / 0.1 < 0.1
     # the count and argument checks of the C entry point
-    arr = (ekpnp.ChannelParams * 4)(ok, debye, drift, ref)
-    assert ekpnp.lib().ekpnp_check_params(arr, 4, None) == 2
+    arr = (ekpnp.ChannelParams * 6)(ok, debye, drift, ref, flow, slow)
+    assert ekpnp.lib().ekpnp_check_params(arr, 6, None) == 3
     assert ekpnp.lib().ekpnp_check_params(arr, 0, None) == ekpnp.ERR_INVALID
 
 
