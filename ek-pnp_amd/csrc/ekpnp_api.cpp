@@ -1019,6 +1019,33 @@ int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho, const doubl
   return EKPNP_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// One channel's starting fields: the reference's gpu_initialization
+// (LBM.cu:147-160: rho0, u = 0, c+- = 0, psi = the wall potential) for the
+// reference and PB-Picard modes, or uniform c_bulk with seeded noise and the
+// wall concentrations on the wall rows for the cold start
+int initial_fields(ekpnp_ctx* ctx, int ch, int mode, uint64_t seed) {
+  const ekpnp_channel_params& p = ctx->params[ch];
+  const ChanDesc& d = ctx->desc[ch];
+  const size_t nn = (size_t)d.nx * d.ny;
+  std::vector<double> rho(nn, p.rho0), zero(nn, 0.0), cp(nn), cn(nn);
+  if (mode == EKPNP_INIT_REFERENCE || mode == EKPNP_INIT_PB_PICARD) {
+    std::vector<double> psi(nn, p.psi_lo);
+    return ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), zero.data(), zero.data(), psi.data());
+  }
+  const size_t T = ctx->total_nx;
+  ekpnp_cold_field(cp.data(), d.nx, d.ny, p.c_bulk, p.init_noise, &ctx->walls[W_CP_LO * T + d.wall_off],
+                   &ctx->walls[W_CP_HI * T + d.wall_off], seed, ch, 0);
+  ekpnp_cold_field(cn.data(), d.nx, d.ny, p.c_bulk, p.init_noise, &ctx->walls[W_CN_LO * T + d.wall_off],
+                   &ctx->walls[W_CN_HI * T + d.wall_off], seed, ch, 1);
+  return ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), cp.data(), cn.data(), nullptr);
+}
+}  // namespace
+
+extern "C" {
+
 int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
   const ApiRange range_("ekpnp_initialization");
   if (!ctx) return EKPNP_ERR_INVALID;
@@ -1026,28 +1053,8 @@ int ekpnp_initialization(ekpnp_ctx* ctx, int mode, uint64_t seed) {
     return fail(ctx, EKPNP_ERR_INVALID, "unknown initialization mode");
   HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
   for (int ch = 0; ch < ctx->nchan; ++ch) {
-    const ekpnp_channel_params& p = ctx->params[ch];
-    const ChanDesc& d = ctx->desc[ch];
-    const size_t nn = (size_t)d.nx * d.ny;
-    std::vector<double> rho(nn, p.rho0), zero(nn, 0.0), cp(nn), cn(nn);
-    if (mode == EKPNP_INIT_REFERENCE || mode == EKPNP_INIT_PB_PICARD) {
-      // gpu_initialization, LBM.cu:147-160
-      std::vector<double> psi(nn, p.psi_lo);
-      int rc = ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), zero.data(),
-                                zero.data(), psi.data());
-      if (rc) return rc;
-    } else {
-      const size_t T = ctx->total_nx;
-      ekpnp_cold_field(cp.data(), d.nx, d.ny, p.c_bulk, p.init_noise,
-                       &ctx->walls[W_CP_LO * T + d.wall_off], &ctx->walls[W_CP_HI * T + d.wall_off],
-                       seed, ch, 0);
-      ekpnp_cold_field(cn.data(), d.nx, d.ny, p.c_bulk, p.init_noise,
-                       &ctx->walls[W_CN_LO * T + d.wall_off], &ctx->walls[W_CN_HI * T + d.wall_off],
-                       seed, ch, 1);
-      int rc = ekpnp_set_fields(ctx, ch, rho.data(), zero.data(), zero.data(), cp.data(), cn.data(),
-                                nullptr);
-      if (rc) return rc;
-    }
+    int rc = initial_fields(ctx, ch, mode, seed);
+    if (rc) return rc;
   }
   if (mode == EKPNP_INIT_PB_PICARD) {
     // LBM.cu:106-127: psi_old = psi; iterate {Boltzmann c+-, Poisson, relax};
@@ -1092,6 +1099,48 @@ int ekpnp_init_equilibrium(ekpnp_ctx* ctx) {
   ctx->has_pops = true;
   ctx->rebase_step = ctx->steps;
   ctx->origin_step = ctx->steps;
+  return EKPNP_OK;
+}
+
+int ekpnp_restart_channel(ekpnp_ctx* ctx, int channel, int mode, uint64_t seed) {
+  const ApiRange range_("ekpnp_restart_channel");
+  int rc = check_channel(ctx, channel);
+  if (rc) return rc;
+  if (mode != EKPNP_INIT_COLD_NOISE && mode != EKPNP_INIT_REFERENCE)
+    return fail(ctx, EKPNP_ERR_UNSUPPORTED, "restart_channel: cold-noise or reference start only");
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "restart_channel: no populations (use init_equilibrium)");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  const ChanDesc& d = ctx->desc[channel];
+  if (!d.active || d.swap) return fail(ctx, EKPNP_ERR_STATE, "restart_channel: channel frozen");
+  // the channel's fields (macroscopic planes, charge, its own Poisson solve)
+  rc = initial_fields(ctx, channel, mode, seed);
+  if (rc) return rc;
+  // every other channel keeps its state: the lattice goes to the unshifted
+  // pre-collision form (a copy of the values, as for set_populations), then
+  // only this channel's work-groups re-derive its references and write its
+  // equilibrium populations
+  if (!ctx->state_q) {
+    HIP_OR_FAIL(ctx, launch_to_q(ctx->opt.precision, ctx->sb, ctx->lat[ctx->cur], ctx->lat[ctx->cur ^ 1], ctx->stream));
+    ctx->cur ^= 1;
+    ctx->state_q = true;
+  }
+  StepBuffers b = ctx->sb;
+  const int t0 = ctx->first_tile[channel];
+  const int t1 = channel + 1 < ctx->nchan ? ctx->first_tile[channel + 1] : (int)ctx->lbm_tiles.size();
+  b.lbm_tiles = ctx->d_lbm_tiles + t0;
+  b.n_lbm_tiles = t1 - t0;
+  int r0 = 0;
+  for (int ch = 0; ch < channel; ++ch) r0 += ctx->desc[ch].ny;
+  b.row_tiles = ctx->d_row_tiles + r0;
+  b.n_row_tiles = d.ny;
+  if (ctx->opt.precision == EKPNP_POP_FP32)
+    HIP_OR_FAIL(ctx, launch_row_means(b, ctx->d_refs, ctx->d_blocks, ctx->stream));
+  HIP_OR_FAIL(ctx, launch_init_equilibrium(ctx->opt.precision, b, ctx->lat[ctx->cur], ctx->stream));
+  // the residual snapshot of this channel restarts from its new u_x
+  const size_t nn = (size_t)d.nx * d.ny;
+  HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->d_ux_prev + d.fld_off, ctx->d_macro + M_UX * ctx->total_nodes + d.fld_off,
+                                  nn * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
   return EKPNP_OK;
 }
 

@@ -97,6 +97,7 @@ SIGNATURES = [
     ("ekpnp_set_wall_profile", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP]),
     ("ekpnp_set_fields", ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP, _DP, _DP, _DP]),
     ("ekpnp_init_equilibrium", ctypes.c_int, [_VP]),
+    ("ekpnp_restart_channel", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_uint64]),
     ("ekpnp_stream_collide_save", ctypes.c_int, [_VP]),
     ("ekpnp_fast_poisson", ctypes.c_int, [_VP]),
     ("ekpnp_poisson_iterative", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -347,6 +348,11 @@ class Solver:
 
     def init_equilibrium(self) -> None:
         self._check(lib().ekpnp_init_equilibrium(self._ctx))
+
+    def restart_channel(self, ch: int, mode: int = COLD_NOISE, seed: int = 0) -> None:
+        """Restart channel ch alone from its starting fields at equilibrium
+        (ekpnp_restart_channel); the other channels keep their state."""
+        self._check(lib().ekpnp_restart_channel(self._ctx, ch, mode, seed))
 
     # -- stepping ---------------------------------------------------------
     def stream_collide_save(self) -> None:

@@ -232,6 +232,15 @@ int ekpnp_set_fields(ekpnp_ctx* ctx, int channel, const double* rho,
                      const double* ux, const double* uy, const double* cpos,
                      const double* cneg, const double* psi);
 int ekpnp_init_equilibrium(ekpnp_ctx* ctx);
+/* Restart one channel from its starting fields (EKPNP_INIT_COLD_NOISE with
+ * `seed`, or EKPNP_INIT_REFERENCE; the fields ekpnp_initialization gives it)
+ * at equilibrium, leaving every other channel's state as it is: the recovery
+ * for a channel that ekpnp_run_to_residual reported non-finite.  Its FP32
+ * storage references are re-derived from the new fields and its residual
+ * snapshot restarts; the step count and the re-centring schedule are the
+ * context's.  No counterpart in the reference (its CUDA errors end the
+ * process, LBM.cu:44-62).  Needs populations (after init_equilibrium). */
+int ekpnp_restart_channel(ekpnp_ctx* ctx, int channel, int mode, uint64_t seed);
 
 #define EKPNP_REBASE_INTERVAL 4096
 

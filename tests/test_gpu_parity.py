@@ -579,6 +579,76 @@ def test_nonfinite_population_is_reported_for_its_channel():
     assert red[1, 6] > 0 and red[0, 6] == 0
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_restart_channel_recovers_only_that_channel(precision):
+    # recovery after a non-finite report: ekpnp_restart_channel restarts
+    # channel 1 alone.  A twin context that was never corrupted, and that
+    # restarts its channel 1 at the same step, must match the recovered one
+    # bit for bit in every channel (the corruption leaves no trace; channels
+    # 0 and 2 go on untouched); and channel 1 must follow a fresh context's
+    # channel 1 from the same seed: bit for bit with FP64 storage, within
+    # 1e-5 with FP32 storage (its re-centring steps are the context's, not a
+    # fresh start's)
+    ps = [ekpnp.lattice_params(64, 40, tau=0.8, zeta=-1.0, kappa_h=5.0, e_x=1e-4),
+          ekpnp.lattice_params(96, 36, tau=0.7, zeta=-2.0, kappa_h=4.0, e_x=2e-4),
+          ekpnp.lattice_params(128, 24, tau=0.9, zeta=-0.5, kappa_h=3.0, e_x=5e-5)]
+    print("case restart_channel", precision, flush=True)
+
+    def start():
+        s = ekpnp.Solver(ps, precision=precision)
+        s.initialization(ekpnp.COLD_NOISE, 7)
+        s.init_equilibrium()
+        return s
+
+    a, b = start(), start()
+    a.step(50)
+    b.step(50)
+    h = a.get_populations(1, "h")
+    h[2, 10, 5] = np.nan
+    a.set_populations(1, "h", h)
+    b.set_populations(1, "h", b.get_populations(1, "h"))
+    with pytest.raises(ekpnp.EkpnpError) as ei:
+        a.run_to_residual(1e-12, 20, 200)
+    assert ei.value.code == ekpnp.ERR_UNSTABLE and "channel 1" in str(ei.value)
+    assert a.steps_done == 70
+    b.step(20)
+    a.restart_channel(1, ekpnp.COLD_NOISE, 7)
+    b.restart_channel(1, ekpnp.COLD_NOISE, 7)
+    a.step(100)
+    b.step(100)
+    fields = ("rho", "ux", "uy", "cpos", "cneg", "psi")
+    for ch in range(3):
+        for f in fields:
+            assert np.array_equal(a.get_field(ch, f), b.get_field(ch, f)), (ch, f)
+        for sp in ("f", "h", "g"):
+            assert np.array_equal(a.get_populations(ch, sp), b.get_populations(ch, sp)), (ch, sp)
+    red = a.reductions()
+    assert (red[:, 6] == 0).all()
+    c = start()
+    c.step(100)
+    for f in fields:
+        e = rel_l2(a.get_field(1, f), c.get_field(1, f))
+        if precision == "fp64":
+            assert np.array_equal(a.get_field(1, f), c.get_field(1, f)), (f, e)
+        else:
+            assert e < 1e-5, (f, e)
+    for s in (a, b, c):
+        s.close()
+    # arguments and state
+    d = ekpnp.Solver(ps[:1], precision=precision)
+    d.initialization(ekpnp.COLD_NOISE, 1)
+    with pytest.raises(ekpnp.EkpnpError) as ei:
+        d.restart_channel(0)
+    assert ei.value.code == ekpnp.ERR_STATE
+    d.init_equilibrium()
+    with pytest.raises(ekpnp.EkpnpError) as ei:
+        d.restart_channel(0, ekpnp.PB_PICARD, 0)
+    assert ei.value.code == ekpnp.ERR_UNSUPPORTED
+    with pytest.raises(ekpnp.EkpnpError):
+        d.restart_channel(1)
+    d.close()
+
+
 @pytest.mark.parametrize("nx,ny,precision", [(8, 4, "fp64"), (2, 5, "fp64"), (3, 4, "fp64"), (8192, 6, "fp64"),
                                              (512, 2050, "fp32"), (256, 4, "fp32"), (24, 7, "fp32")])
 def test_extreme_shapes(nx, ny, precision):
