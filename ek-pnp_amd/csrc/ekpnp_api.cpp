@@ -371,11 +371,11 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   if (e != hipSuccess) return e;
   e = hipMemcpy(c->d_desc, c->desc.data(), c->nchan * sizeof(ChanDesc), hipMemcpyHostToDevice);
   if (e != hipSuccess) return e;
-  // chunk-entry Thomas coefficients of the new chunking
-  for (int i = 0; i < c->nchan; ++i) {
-    e = launch_cprime_table(c->sb, i, c->desc[i].tri_C, c->desc[i].nk, c->stream);
-    if (e != hipSuccess) return e;
-  }
+  // chunk-entry Thomas coefficients of the new chunking (one launch)
+  long long max_entries = 0;
+  for (const auto& d : c->desc) max_entries = std::max(max_entries, (long long)d.tri_C * d.nk);
+  e = launch_cprime_tables(c->sb, c->nchan, max_entries, c->stream);
+  if (e != hipSuccess) return e;
   c->sb.fft_tiles = c->d_fft_tiles;
   c->sb.n_fft_tiles = (int)c->fft_tiles.size();
   c->sb.n_fft_pow2 = n_fft_pow2;

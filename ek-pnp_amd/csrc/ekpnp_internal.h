@@ -139,7 +139,9 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
 // not, lds bytes of dynamic LDS each (registers and LDS both counted); 0 on error
 int fft_one_tile_blocks_per_cu(bool twl, unsigned lds);
 // chunk-entry Thomas coefficients of channel chan for its current tri_C, tri_L
-hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s);
+// chunk-entry Thomas coefficients of every channel (max_entries: the
+// largest tri_C x nk over the channels)
+hipError_t launch_cprime_tables(const StepBuffers& b, int n_channels, long long max_entries, hipStream_t s);
 hipError_t launch_init_equilibrium(int precision, const StepBuffers& b, void* dst, hipStream_t s);
 // pre-collision populations of the next step, written unshifted into dst
 hipError_t launch_to_q(int precision, const StepBuffers& b, const void* src, void* dst,

@@ -2017,13 +2017,15 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag_walls(StepBuffers b, const f
 
 // table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
 // j L is the last row of chunk j-1); row 0 of the table is unused
-__global__ __launch_bounds__(256) void k_cprime_table(StepBuffers b, int chan) {
-  const ChanDesc* __restrict__ cd = b.chans + chan;
+// every channel's table in one launch: channel chan0 + blockIdx.y, its
+// C x nk entries grid-strided over x
+__global__ __launch_bounds__(256) void k_cprime_table(StepBuffers b, int chan0) {
+  const ChanDesc* __restrict__ cd = b.chans + chan0 + (int)blockIdx.y;
   const int nk = cd->nk, C = cd->tri_C, L = cd->tri_L;
-  const int t = blockIdx.x * 256 + (int)threadIdx.x;
-  if (t >= C * nk) return;
-  const int j = t / nk, k = t - j * nk;
-  b.cprime[cd->cp_off + t] = j == 0 ? 0.0 : cprime_closed(j * L, (double)k * cd->kdy_scale);
+  for (int t = blockIdx.x * 256 + (int)threadIdx.x; t < C * nk; t += (int)gridDim.x * 256) {
+    const int j = t / nk, k = t - j * nk;
+    b.cprime[cd->cp_off + t] = j == 0 ? 0.0 : cprime_closed(j * L, (double)k * cd->kdy_scale);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2430,8 +2432,12 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_cprime_table(const StepBuffers& b, int chan, int tri_C, int nk, hipStream_t s) {
-  hipLaunchKernelGGL(k_cprime_table, dim3((tri_C * nk + 255) / 256), dim3(256), 0, s, b, chan);
+hipError_t launch_cprime_tables(const StepBuffers& b, int n_channels, long long max_entries, hipStream_t s) {
+  const int gx = (int)std::min<long long>(64, std::max<long long>(1, (max_entries + 255) / 256));
+  for (int c0 = 0; c0 < n_channels; c0 += 65535) {
+    const int nc = std::min(65535, n_channels - c0);
+    hipLaunchKernelGGL(k_cprime_table, dim3(gx, nc), dim3(256), 0, s, b, c0);
+  }
   return hipGetLastError();
 }
 
