@@ -287,6 +287,9 @@ def main():
         stage_nodes = sum(p.nx * (p.ny - 2) if p.nx % 64 == 0 else p.nx * p.ny for p in params)
         kernel = ("k_lbm_rows<false>: fused collide-stream of the interior rows "
                   "(the two wall rows ride on the tridiagonal launch)")
+        if any(p.nx % 64 for p in params):
+            kernel = ("LBM stage: k_lbm_rows<false> (interior rows of the channels with nx a multiple of 64; "
+                      "their wall rows ride on the tridiagonal launch) + k_lbm (every row of the others)")
     else:
         stage_nodes = nodes
         kernel = ("LBM stage: k_lbm_rows<false> (interior rows) + k_lbm_rows<true> (wall rows), "
