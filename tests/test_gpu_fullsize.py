@@ -13,7 +13,7 @@ import pytest
 
 import ekpnp
 import oracle as O
-from helpers import field_errors
+from helpers import assert_parity, field_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +67,53 @@ def test_wall_profile_channel_8192x2048_against_the_oracle():
     print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     for f in ("rho", "ux", "cpos", "cneg", "psi"):
         assert errs[f] < 1e-5, (f, errs[f])
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_batched_sweep_against_the_oracle(precision):
+    # the 256-channel batched sweep (seed 2: 18.2 M nodes, nx 32..256, ny
+    # 64..1024; row-kernel and general-kernel channels, the wall rows on the
+    # tridiagonal launch) 32 steps from the seeded cold start (FP32
+    # re-centrings at 8, 16, 32 included), every channel against the oracle
+    # started from the same state.  FP64 storage: within 1e-11 (max-norm with
+    # a cancellation floor).  FP32 storage: rho, c+-, psi within 1e-5 relative
+    # L2 on every channel; u_x, which starts at rest and is still ~1e-6 here
+    # on channels with a weak applied field, is quantised by the FP32 storage
+    # at ~1e-5 of itself through the first ~1000 steps of a cold start
+    # (channel 0, E_x 1.5e-5: 1.5e-5 from step 16 to 512, 8.7e-6 at 1024,
+    # 1.6e-6 at 8192; the 5000-step comparison of this sweep against FP64
+    # storage stays within 1e-5 on every channel), so here it is bounded at
+    # 3e-5 and within 1e-5 on at least 90 % of the channels
+    from ekpnp import surrogate as S
+
+    print("case batched sweep vs oracle", precision, flush=True)
+    ps, _ = S.sample_channels(256, 2)
+    s = ekpnp.Solver(ps, precision=precision)
+    s.initialization(ekpnp.COLD_NOISE, 2)
+    s.init_equilibrium()
+    s.step(32)
+    errs = []
+    for ch, p in enumerate(ps):
+        o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
+        o.initialization(ekpnp.COLD_NOISE, 2, ch)
+        o.init_equilibrium()
+        o.step(32)
+        if precision == "fp64":
+            assert_parity(s, ch, o, 1e-11, tag=f"sweep ch{ch} {p.nx}x{p.ny}")
+        else:
+            errs.append(field_errors(s, ch, o, fields=("rho", "ux", "cpos", "cneg", "psi")))
+        o.close()
+    s.close()
+    if precision == "fp32":
+        for f in ("rho", "ux", "cpos", "cneg", "psi"):
+            e = np.array([r[f] for r in errs])
+            print(f"{f}: largest {e.max():.2e} (channel {int(e.argmax())}), median {np.median(e):.2e}, "
+                  f"{int((e >= 1e-5).sum())} of {len(e)} at or above 1e-5", flush=True)
+        for f in ("rho", "cpos", "cneg", "psi"):
+            assert max(r[f] for r in errs) < 1e-5, f
+        ux = np.array([r["ux"] for r in errs])
+        assert ux.max() < 3e-5 and (ux < 1e-5).mean() >= 0.9
 
 
 def test_bench_channel_conserves_fluid_mass():
