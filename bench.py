@@ -222,6 +222,9 @@ def main():
                     help="collision arithmetic for FP32 population storage")
     ap.add_argument("--tune-reps", type=int, default=10,
                     help="Poisson solves per candidate plan in the autotuner (0: default plan)")
+    ap.add_argument("--plan", default=None, metavar="FFT_ELEMS,TRI_ROWS,FAMILY",
+                    help="use this Poisson plan instead of autotuning (family 0 one-tile, 1 persistent, "
+                         "2 row-pair), e.g. to profile the kernels the autotuner picked")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -241,7 +244,12 @@ def main():
     print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     s = ekpnp.Solver(params, precision=precision, compute_fp64=(args.compute == "fp64"))
-    plan = s.autotune(args.tune_reps) if args.tune_reps > 0 else None
+    if args.plan:
+        fe, tr, fam = (int(v) for v in args.plan.split(","))
+        s.set_poisson_plan(fe, tr, fam)
+        plan = s.poisson_plan()
+    else:
+        plan = s.autotune(args.tune_reps) if args.tune_reps > 0 else None
     apply_walls(s, args.config, params, seed)
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()

@@ -317,6 +317,21 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   unsigned lds_pow2 = 16, lds_dft = 16, lds_pow2_tw = 16;  // rows; rows + quadrant table
   unsigned lds_pow2_t2 = 16;                                 // rows + two-level table
   int n_fft_pow2 = 0;
+  // wide row-pair kernels: every power-of-two channel has rows of
+  // N = FFT_PAIR_WIDE_S * FFT_WG_ROWS complex values (nx = 8192), asked for
+  // as fft_elems = N with the row-pair family; tiles of two rows, one row
+  // buffer and the two-level twiddle table in LDS
+  constexpr int kWideN = FFT_PAIR_WIDE_S * FFT_WG_ROWS;
+  bool wide = fft_persist == 2 && fft_elems == kWideN;
+  int n_pow2_chan = 0;
+  for (const auto& d : c->desc)
+    if (d.pow2) {
+      ++n_pow2_chan;
+      wide = wide && d.nx == 2 * kWideN;
+    }
+  wide = wide && n_pow2_chan > 0;
+  const unsigned wide_lds = (unsigned)((kWideN + tw2_count(2 * kWideN)) * sizeof(double2));
+  wide = wide && wide_lds <= (unsigned)c->max_lds;
   for (int pass = 0; pass < 2; ++pass) {
     for (int i = 0; i < c->nchan; ++i) {
       ChanDesc& d = c->desc[i];
@@ -325,12 +340,12 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
       int rows = 1;
       if (d.pow2) {
         const int N = d.nx / 2;
-        rows = std::max(1, std::min(std::min(M, FFT_MAX_ROWS), fft_elems / N));
+        rows = wide ? 2 : std::max(1, std::min(std::min(M, FFT_MAX_ROWS), fft_elems / N));
         // rows are padded to whole 16-element groups (LDS swizzle)
         const int padded = (rows * N + 15) & ~15;
         lds_pow2 = std::max<unsigned>(lds_pow2, (unsigned)(padded * sizeof(double2)));
         lds_pow2_tw = std::max<unsigned>(lds_pow2_tw, (unsigned)((padded + d.nx / 4) * sizeof(double2)));
-        const int t2 = 64 + std::max(1, d.nx / 64);  // the kernels' two-level twiddle table
+        const int t2 = tw2_count(d.nx);  // the kernels' two-level twiddle table
         lds_pow2_t2 = std::max<unsigned>(lds_pow2_t2, (unsigned)((padded + t2) * sizeof(double2)));
       } else {
         // direct-DFT tiles: as many whole rows as fit fft_elems values and
@@ -408,7 +423,11 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
   // N = S * FFT_WG_ROWS, S in {2, 4}, two rows per tile (fft_elems = 2 N),
   // rows and twiddles in LDS
   c->sb.fft_pair_s = 0;
-  if (fft_persist == 2 && n_fft_pow2 > 0 && c->sb.fft_tw_lds) {
+  if (wide) {
+    c->sb.fft_pair_s = FFT_PAIR_WIDE_S;
+    c->sb.fft_tw_lds = 0;
+    c->sb.fft_lds_pow2 = wide_lds;
+  } else if (fft_persist == 2 && n_fft_pow2 > 0 && c->sb.fft_tw_lds) {
     int n_common = -1;
     bool ok = true;
     for (const auto& d : c->desc) {

@@ -21,6 +21,11 @@ constexpr int FFT_WG = 256;       // lanes of a persistent x-FFT work-group
 constexpr int FFT_WG_ROWS = 512;  // lanes of a one-tile x-FFT work-group (radix-8 passes)
 constexpr int FFT_MAX_ELEMS = 4096;  // complex values held in LDS by one FFT work-group
 constexpr int FFT_ROW_SLOTS = FFT_MAX_ELEMS / FFT_WG_ROWS;  // values per lane of a one-tile load
+// two-level twiddle table of the x-FFTs: 64 fine values W^f, then
+// max(1, nx / 64) coarse values W^(64 c); at most 192 (3 KB) for nx <= 8192
+constexpr int TW2_LGF = 6, TW2_FINE = 1 << TW2_LGF;
+__host__ __device__ constexpr int tw2_count(int nx) { return TW2_FINE + (nx / TW2_FINE > 1 ? nx / TW2_FINE : 1); }
+constexpr int FFT_PAIR_WIDE_S = 8;  // values per lane and row of the wide row-pair kernels (nx = 8192)
 constexpr int FFT_MAX_ROWS = 256;  // rows per power-of-two FFT tile
 // spectrum pairs (X[p], X[N-p]) per lane: rows (N/2 + 1) <= FFT_MAX_ELEMS/2 + FFT_MAX_ROWS
 constexpr int FFT_PAIR_SLOTS = (FFT_MAX_ELEMS / 2 + FFT_MAX_ROWS + FFT_WG_ROWS - 1) / FFT_WG_ROWS;
@@ -95,6 +100,8 @@ struct StepBuffers {
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
   int fft_persist;              // power-of-two tiles by the persistent kernels
   int fft_pair_s;               // > 0: by the row-pair kernels, N = fft_pair_s * FFT_WG_ROWS for every one
+                                //   (2, 4: two row buffers and the quadrant table; FFT_PAIR_WIDE_S: one row
+                                //   buffer, the second row in registers, the two-level table)
   int fft_grid;                 // work-groups of the persistent kernels
   unsigned fft_lds_pow2;        // dynamic LDS bytes of the Stockham kernels
   unsigned fft_lds_dft;         // dynamic LDS bytes of the direct-DFT kernels

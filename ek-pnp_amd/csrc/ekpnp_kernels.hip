@@ -1026,14 +1026,15 @@ __device__ __forceinline__ void dft_reg(double2* v) {
   }
 }
 
-// e^{SIGN 2 pi i t/nx} for 0 <= t < nx from the quadrant table
+// e^{SIGN 2 pi i t/nx} for 0 <= t < nx from w = q[tw_quad_index(t, lgnx)],
+// the quadrant-table value of t
+__device__ __forceinline__ int tw_quad_index(int t, int lgnx) { return t & ((1 << (lgnx - 2)) - 1); }
+
 template <int SIGN>
-__device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t, int lgnx) {
+__device__ __forceinline__ double2 tw_quad_resolve(double2 w, int t, int lgnx) {
   // t in [0, nx): quadrant 0..3 multiplies the table value by (-i)^quad:
   // (x, y), (y, -x), (-x, -y), (-y, x); SIGN > 0 conjugates
-  const int sh = lgnx - 2;
-  const int quad = t >> sh;
-  const double2 w = q[t & ((1 << sh) - 1)];
+  const int quad = t >> (lgnx - 2);
   const bool swap = quad & 1;
   const double a = swap ? w.y : w.x, b = swap ? w.x : w.y;
   const bool nre = (quad & 2) != 0;
@@ -1041,13 +1042,17 @@ __device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t,
   return make_double2(neg_if(a, nre), neg_if(b, nim));
 }
 
+// e^{SIGN 2 pi i t/nx} for 0 <= t < nx from the quadrant table
+template <int SIGN>
+__device__ __forceinline__ double2 tw_full(const double2* __restrict__ q, int t, int lgnx) {
+  return tw_quad_resolve<SIGN>(q[tw_quad_index(t, lgnx)], t, lgnx);
+}
+
 // Two-level twiddle table, for rows whose quadrant table (nx / 4 values)
 // would cost a resident work-group of LDS: F[f] = W^f for f < 64, then
 // C[c] = W^(64 c) for c < max(1, nx / 64), W = e^{-2 pi i / nx}, both taken
 // from the quadrant table; W^t = C[t >> 6] F[t & 63], one complex product
-// (exact where either factor is 1).  At most 192 values (3 KB) for nx <= 8192.
-constexpr int TW2_LGF = 6, TW2_FINE = 1 << TW2_LGF;
-__host__ __device__ constexpr int tw2_count(int nx) { return TW2_FINE + (nx / TW2_FINE > 1 ? nx / TW2_FINE : 1); }
+// (exact where either factor is 1).  tw2_count(nx) values.
 
 template <int WGT>
 __device__ __forceinline__ void tw2_fill(double2* t2, const double2* __restrict__ q, int nx, int lgnx) {
@@ -1823,6 +1828,176 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv2(StepBuffers b, const i
 }
 
 // ---------------------------------------------------------------------------
+// Row-pair x-FFTs for rows too wide for two row buffers per work-group
+// (nx = 8192: two 64 KB rows would leave one work-group per CU).  One row
+// buffer and the two-level twiddle table in LDS (67 KB, two work-groups per
+// CU); row B waits in registers while row A is transformed and moves into the
+// buffer once every lane has read row A's result out of it.  Load order: the
+// twiddle table's quadrant-table values, row A, row B, so that filling the
+// table waits for its own loads only and row A's first use for row A's.  The
+// wall lifts of rows 1 and ny-2 are read where they are applied, by the two
+// tiles holding those rows.  Same arithmetic as the one-tile kernels with the
+// two-level table.  Four waves per SIMD (two work-groups per CU, the LDS
+// limit) cap the registers at 128.
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_fwd2w(StepBuffers b, const int4* __restrict__ tiles,
+                                                              const ChanDesc* __restrict__ chans) {
+  extern __shared__ double2 lds[];
+  constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, nx = 2 * N, quarter = N / 2;
+  constexpr int NT2 = tw2_count(nx);
+  static_assert(NT2 <= WG && N % 16 == 0, "wide row-pair FFT sizes");
+  const int4 t = tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
+  if (!cd->active) return;
+  const int ny = cd->ny, y0 = t.y;
+  const bool two = t.z == 2;
+  const double* __restrict__ re = b.rho_e + cd->fld_off;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  double2* __restrict__ spec = b.spec + cd->spec_off;
+  const double pc = cd->pc_dy2;
+  const int tid = threadIdx.x;
+  double2* x = lds;
+  double2* t2 = lds + N;
+  const int i2 = min(tid, NT2 - 1);
+  const int tt = (i2 < TW2_FINE ? i2 : (i2 - TW2_FINE) << TW2_LGF) & (nx - 1);
+  const double2 wq = tw[tw_quad_index(tt, lgnx)];
+  const double2* __restrict__ srcA = reinterpret_cast<const double2*>(re + (long long)y0 * nx);
+  const double2* __restrict__ srcB = reinterpret_cast<const double2*>(re + (long long)(y0 + (two ? 1 : 0)) * nx);
+  double2 va[S], vb[S];
+  // the three groups of loads stay in this order (no instruction is moved
+  // across a scheduling barrier), so each wait below counts only the loads
+  // issued after the ones it needs
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < S; ++c) va[c] = srcA[tid + c * WG];
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int c = 0; c < S; ++c) vb[c] = srcB[tid + c * WG];
+  __builtin_amdgcn_sched_barrier(0);
+  // every lane writes (lanes past the table rewrite its last value), so the
+  // table's load is not moved into a branch behind the rows' loads
+  t2[i2] = tw_quad_resolve<-1>(wq, tt, lgnx);
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1 && !two) break;
+    if (r == 1) __syncthreads();  // row A's split has read the row buffer
+    const int y = y0 + r;
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const double2 v = r == 0 ? va[c] : vb[c];
+      x[sw(tid + c * WG)] = make_double2(-pc * v.x, -pc * v.y);
+    }
+    // wall lifts (odd_extension, poisson.cu:229-241): row 1 takes psi_lo,
+    // row ny-2 psi_hi (distinct rows: ny >= 4); each lane updates its own
+    // elements, so the barrier below orders them for the transform
+    if (y == 1 || y == ny - 2) {
+      const double2* __restrict__ w = reinterpret_cast<const double2*>(
+          b.walls + cd->wall_off + (y == 1 ? W_PSI_LO : W_PSI_HI) * b.walls_stride);
+#pragma unroll
+      for (int c = 0; c < S; ++c) {
+        const int e = tid + c * WG;
+        const double2 l = w[e], v = x[sw(e)];
+        x[sw(e)] = make_double2(v.x - l.x, v.y - l.y);
+      }
+    }
+    __syncthreads();
+    stockham<-1, WG, 8, true>(x, 1, N, lgN, t2, lgnx);
+    // split X[k] = E[k] + W^k O[k], k = 0..N (lanes past N repeat k = N)
+#pragma unroll
+    for (int c = 0; c <= S; ++c) {
+      const int k = min(tid + c * WG, N);
+      const double2 zk = x[sw(k & (N - 1))];
+      const double2 zc = cconj(x[sw((N - k) & (N - 1))]);
+      const double2 E = make_double2(0.5 * (zk.x + zc.x), 0.5 * (zk.y + zc.y));
+      const double2 dd = csub(zk, zc);
+      const double2 O = make_double2(0.5 * dd.y, -0.5 * dd.x);
+      spec[spec_idx(ny, y, k)] = cadd(E, cmul(tw_half_get<true>(t2, k, quarter, lgnx), O));
+    }
+  }
+}
+
+template <int S>
+__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_inv2w(StepBuffers b, const int4* __restrict__ tiles,
+                                                              const ChanDesc* __restrict__ chans) {
+  extern __shared__ double2 lds[];
+  constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, nx = 2 * N, quarter = N / 2;
+  constexpr int half = N / 2, P = half / WG + 1, NT2 = tw2_count(nx);  // pair slots per lane
+  static_assert(NT2 <= WG && N % 16 == 0, "wide row-pair FFT sizes");
+  const int4 t = tiles[blockIdx.x];
+  const ChanDesc* __restrict__ cd = chans + t.x;
+  if (!cd->active) return;
+  const int ny = cd->ny, y0 = t.y;
+  const bool two = t.z == 2;
+  const double2* __restrict__ tw = b.tw + cd->tw_off;
+  const double2* __restrict__ spec = b.spec + cd->spec_off;
+  double* __restrict__ ps = b.psi + cd->fld_off;
+  const int tid = threadIdx.x;
+  double2* x = lds;
+  double2* t2 = lds + N;
+  const int i2 = min(tid, NT2 - 1);
+  const int tt = (i2 < TW2_FINE ? i2 : (i2 - TW2_FINE) << TW2_LGF) & (nx - 1);
+  const double2 wq = tw[tw_quad_index(tt, lgnx)];
+  // pairs X[p], X[N - p], p = 0..N/2 (lanes past N/2 repeat p = N/2); the
+  // table's load, row A's and row B's stay in this order (scheduling barriers)
+  double2 xa[2][P], xb[2][P];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    __builtin_amdgcn_sched_barrier(0);
+    const int y = y0 + (r == 1 && two ? 1 : 0);
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int p = min(tid + c * WG, half);
+      xa[r][c] = spec[spec_idx(ny, y, p)];
+      xb[r][c] = spec[spec_idx(ny, y, N - p)];
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  t2[i2] = tw_quad_resolve<-1>(wq, tt, lgnx);  // every lane: no branch around the load
+  __syncthreads();  // twiddles in LDS before the merge reads them
+  const double invN = 1.0 / (double)N;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    if (r == 1 && !two) break;
+    if (r == 1) __syncthreads();  // row A's output has been read from the row buffer
+    // z[k] = E + i O, E = (X[k] + conj X[N-k]) / 2,
+    // O = (X[k] - conj X[N-k]) / 2 * conj W^k, for k = p and k = N - p
+#pragma unroll
+    for (int c = 0; c < P; ++c) {
+      const int p = min(tid + c * WG, half);
+      {
+        const double2 xk = xa[r][c], xc = cconj(xb[r][c]);
+        const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+        const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                               cconj(tw_half_get<true>(t2, p, quarter, lgnx)));
+        x[sw(p)] = make_double2(E.x - O.y, E.y + O.x);
+      }
+      if (p > 0 && p < half) {
+        const double2 xk = xb[r][c], xc = cconj(xa[r][c]);
+        const double2 E = make_double2(0.5 * (xk.x + xc.x), 0.5 * (xk.y + xc.y));
+        const double2 O = cmul(make_double2(0.5 * (xk.x - xc.x), 0.5 * (xk.y - xc.y)),
+                               cconj(tw_half_get<true>(t2, N - p, quarter, lgnx)));
+        x[sw(N - p)] = make_double2(E.x - O.y, E.y + O.x);
+      }
+    }
+    __syncthreads();
+    stockham<+1, WG, 8, true>(x, 1, N, lgN, t2, lgnx);
+    double2* __restrict__ dst = reinterpret_cast<double2*>(ps + (long long)(y0 + r) * nx);
+#pragma unroll
+    for (int c = 0; c < S; ++c) {
+      const double2 z = x[sw(tid + c * WG)];
+      dst[tid + c * WG] = make_double2(z.x * invN, z.y * invN);
+    }
+  }
+  // Dirichlet rows psi(0) = voltage, psi(ny-1) = voltage2 (odd_extract,
+  // poisson.cu:297-303), written by the tiles holding rows 1 and ny-2
+  const double* __restrict__ Wl = b.walls + cd->wall_off;
+  const long long ws = b.walls_stride;
+  if (y0 == 1) copy_row_batched<WG>(ps, Wl + W_PSI_LO * ws, nx);
+  if (y0 + t.z - 1 == ny - 2) copy_row_batched<WG>(ps + (long long)(ny - 1) * nx, Wl + W_PSI_HI * ws, nx);
+}
+
+// ---------------------------------------------------------------------------
 // k_tridiag: for each wavenumber k the system
 //   x_{i-1} + a x_i + x_{i+1} = r_i,  i = 1..M,  x_0 = x_{M+1} = 0,
 //   a = -(2 + (kx dy)^2) = -2 cosh(theta)
@@ -2389,7 +2564,10 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_pow2 > 0 && b.fft_pair_s > 0) {
     const dim3 grid(b.n_fft_pow2), block(FFT_WG_ROWS);
     if (b.fft_pair_s == 2) hipLaunchKernelGGL(k_fft_fwd2<2>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
-    else hipLaunchKernelGGL(k_fft_fwd2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else if (b.fft_pair_s == 4) hipLaunchKernelGGL(k_fft_fwd2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else if (b.fft_pair_s == FFT_PAIR_WIDE_S)
+      hipLaunchKernelGGL(k_fft_fwd2w<FFT_PAIR_WIDE_S>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else return hipErrorInvalidValue;
   } else if (b.n_fft_pow2 > 0 && b.fft_persist) {
     hipLaunchKernelGGL(k_fft_fwd_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
                        b.n_fft_pow2);
@@ -2416,7 +2594,10 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
   if (b.n_fft_pow2 > 0 && b.fft_pair_s > 0) {
     const dim3 grid(b.n_fft_pow2), block(FFT_WG_ROWS);
     if (b.fft_pair_s == 2) hipLaunchKernelGGL(k_fft_inv2<2>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
-    else hipLaunchKernelGGL(k_fft_inv2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else if (b.fft_pair_s == 4) hipLaunchKernelGGL(k_fft_inv2<4>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else if (b.fft_pair_s == FFT_PAIR_WIDE_S)
+      hipLaunchKernelGGL(k_fft_inv2w<FFT_PAIR_WIDE_S>, grid, block, b.fft_lds_pow2, s, b, b.fft_tiles, b.chans);
+    else return hipErrorInvalidValue;
   } else if (b.n_fft_pow2 > 0 && b.fft_persist) {
     hipLaunchKernelGGL(k_fft_inv_p, dim3(b.fft_grid), dim3(FFT_WG), FFT_LDS_PERSIST, s, b, b.fft_tiles, b.chans,
                        b.n_fft_pow2);
@@ -2433,7 +2614,10 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s) {
 }
 
 hipError_t launch_cprime_tables(const StepBuffers& b, int n_channels, long long max_entries, hipStream_t s) {
-  const int gx = (int)std::min<long long>(64, std::max<long long>(1, (max_entries + 255) / 256));
+  // one work-group per 256 entries of the largest table, about 4096 in all
+  // (the kernel strides over the rest)
+  const long long cap = std::max<long long>(8, 4096 / std::max(1, n_channels));
+  const int gx = (int)std::min<long long>(cap, std::max<long long>(1, (max_entries + 255) / 256));
   for (int c0 = 0; c0 < n_channels; c0 += 65535) {
     const int nc = std::min(65535, n_channels - c0);
     hipLaunchKernelGGL(k_cprime_table, dim3(gx, nc), dim3(256), 0, s, b, c0);

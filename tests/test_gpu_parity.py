@@ -139,12 +139,15 @@ def test_poisson_only(nx, ny):
     assert rel_l2(s.get_field(0, "psi"), want) < 1e-12
 
 
-@pytest.mark.parametrize("nx,ny", [(2048, 7), (4096, 6), (4096, 4), (2048, 34)])
+@pytest.mark.parametrize("nx,ny", [(2048, 7), (4096, 6), (4096, 4), (2048, 34), (8192, 7), (8192, 4), (8192, 10)])
 def test_row_pair_fft_kernels(nx, ny):
     # the row-pair x-FFTs (two rows per work-group, the second row's loads in
     # flight during the first's transform): odd interior row counts leave a
     # one-row last tile (holding the upper wall lift), ny = 4 puts both wall
-    # lifts in one tile
+    # lifts in one tile.  nx = 8192 takes the wide kernels (one row buffer,
+    # the second row in registers, the two-level twiddle table), asked for
+    # with fft_elems = 4096, against the one-tile kernels with one row per
+    # tile and the same table
     rng = np.random.default_rng(nx + ny)
     p = ekpnp.lattice_params(nx, ny, tau=0.8, zeta=-1.0, kappa_h=8.0, e_x=1e-4)
     shp = (ny, nx)
@@ -158,10 +161,11 @@ def test_row_pair_fft_kernels(nx, ny):
     o.set_wall_profile(psi_lo=lo, psi_hi=hi)
     o.set_fields(np.ones(shp), np.zeros(shp), np.zeros(shp), cp, cn)
     want = o.get_field("psi")
-    s.set_poisson_plan(nx, 16, 0)
+    elems = min(nx, 4096)
+    s.set_poisson_plan(elems, 16, 0)
     s.set_fields(0, np.ones(shp), np.zeros(shp), np.zeros(shp), cp, cn)
     one_tile = s.get_field(0, "psi")
-    s.set_poisson_plan(nx, 16, 2)
+    s.set_poisson_plan(elems, 16, 2)
     assert s.poisson_plan()["fft_kernels"] == "row-pair"
     s.fast_poisson()
     pair = s.get_field(0, "psi")
@@ -170,7 +174,7 @@ def test_row_pair_fft_kernels(nx, ny):
     assert e < 1e-12
     assert rel_l2(pair, one_tile) < 1e-14
     # a plan the channels do not allow falls back to the one-tile kernels
-    s.set_poisson_plan(nx // 2, 16, 2)
+    s.set_poisson_plan(elems // 2, 16, 2)
     assert s.poisson_plan()["fft_kernels"] == "one-tile"
     s.fast_poisson()
     assert rel_l2(s.get_field(0, "psi"), want) < 1e-12
@@ -187,14 +191,16 @@ def test_plan_arguments_are_validated():
     assert not plan["walls_in_tri"]  # FP64 storage: no row kernel
 
 
-def test_row_pair_fft_kernels_in_a_coupled_batch():
-    # two 2048-wide channels on the row-pair kernels beside a 10-wide one on
-    # the direct DFT, FP64 storage, against the oracle after 40 steps
-    ps = [ekpnp.lattice_params(2048, 20, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4),
-          ekpnp.lattice_params(2048, 11, tau=0.8, zeta=-1.0, kappa_h=3.0, e_x=1e-4),
+@pytest.mark.parametrize("nx", [2048, 8192])
+def test_row_pair_fft_kernels_in_a_coupled_batch(nx):
+    # two nx-wide channels on the row-pair kernels (nx = 8192: the wide ones)
+    # beside a 10-wide one on the direct DFT, FP64 storage, against the oracle
+    # after 40 steps
+    ps = [ekpnp.lattice_params(nx, 20, tau=0.7, zeta=-1.5, kappa_h=6.0, e_x=2e-4),
+          ekpnp.lattice_params(nx, 11, tau=0.8, zeta=-1.0, kappa_h=3.0, e_x=1e-4),
           ekpnp.lattice_params(10, 16, tau=0.9, zeta=-0.5, kappa_h=3.0, e_x=1e-5)]
     s = ekpnp.Solver(ps, precision="fp64")
-    s.set_poisson_plan(2048, 16, 2)
+    s.set_poisson_plan(min(nx, 4096), 16, 2)
     assert s.poisson_plan()["fft_kernels"] == "row-pair"
     s.initialization(ekpnp.COLD_NOISE, 4)
     s.init_equilibrium()
@@ -204,7 +210,7 @@ def test_row_pair_fft_kernels_in_a_coupled_batch():
         o.initialization(ekpnp.COLD_NOISE, 4, ch)
         o.init_equilibrium()
         o.step(40)
-        assert_parity(s, ch, o, FP64_TOL, tag=f"row-pair batch ch={ch}")
+        assert_parity(s, ch, o, FP64_TOL, tag=f"row-pair batch nx={nx} ch={ch}")
 
 
 def test_batched_channels_match_single():
