@@ -345,10 +345,11 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
  * where the tiles allow them (fft_elems <= 2048, nx <= 4096), 2 the row-pair
  * kernels (two rows per work-group transformed one after the other, the
  * second row's loads in flight during the first's transform) where every
- * power-of-two channel has nx = 2048 or 4096 and fft_elems = nx; otherwise
- * the one-tile kernels.  Results do not depend on fft_elems or the kernel
- * choice; tri_rows changes the FP64 rounding of psi only (chunk
- * boundaries). */
+ * power-of-two channel has nx = 2048 or 4096 and fft_elems = nx, or every
+ * one has nx = 8192 and fft_elems = 4096 (the second row waits in
+ * registers); otherwise the one-tile kernels.  Results do not depend on
+ * fft_elems or the kernel choice; tri_rows changes the FP64 rounding of psi
+ * only (chunk boundaries). */
 int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_persistent);
 /* Times the Poisson stages over these channels for fft_elems in {1024,
  * 2048, 4096}, the three FFT kernel families and tri_rows in {8, 16} (reps solves
