@@ -1,7 +1,8 @@
 """The benchmark workloads at their full sizes against the FP64 oracle.
 
-A few coupled steps of the 4096 x 1024 channel (the bench workload: row-pair
-x-FFTs) and of the 8192 x 2048 channel with zeta(x) walls (one-tile x-FFTs),
+A few coupled steps of the 4096 x 1024 channel (row-pair x-FFTs) and of the
+8192 x 2048 channel with zeta(x) walls (the bench workload: wide row-pair
+x-FFTs, the plan the autotuner picks for it),
 FP32 deviation storage with FP64 arithmetic (the row LBM kernel, its wall rows
 on the tridiagonal launch, the tridiagonal at its real column height),
 compared field by field with the oracle started from the
@@ -27,8 +28,10 @@ def wall_zeta(nx, seed, zeta0=-1.5, modes=4, amp=0.5):
     return zeta0 + sum(a[k] * np.sin(2 * np.pi * (k + 1) * x / nx + phi[k]) for k in range(modes))
 
 
-def _run_pair(p, seed, nsteps, walls=None):
+def _run_pair(p, seed, nsteps, walls=None, plan=None):
     s = ekpnp.Solver([p], precision="fp32")
+    if plan is not None:
+        s.set_poisson_plan(*plan)
     o = O.Oracle(p, poisson_method=O.POISSON_XDFT_THOMAS)
     if walls is not None:
         s.set_wall_profile(0, **walls)
@@ -62,7 +65,10 @@ def test_wall_profile_channel_8192x2048_against_the_oracle():
     p = ekpnp.lattice_params(8192, 2048, tau=0.8, zeta=-1.5, kappa_h=64.0, e_x=1e-4)
     z = wall_zeta(p.nx, 4)
     walls = dict(psi_lo=z, psi_hi=z, cpos_lo=np.exp(-z), cpos_hi=np.exp(-z), cneg_lo=np.exp(z), cneg_hi=np.exp(z))
-    s, o = _run_pair(p, 4, 4, walls)
+    s, o = _run_pair(p, 4, 4, walls, plan=(4096, 16, 2))
+    plan = s.poisson_plan()
+    print("plan", plan, flush=True)
+    assert plan["fft_kernels"] == "row-pair" and plan["walls_in_tri"]
     errs = field_errors(s, 0, o)
     print(" ".join(f"{k}={v:.2e}" for k, v in errs.items()), flush=True)
     for f in ("rho", "ux", "cpos", "cneg", "psi"):
