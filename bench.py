@@ -222,9 +222,10 @@ def main():
                     help="collision arithmetic for FP32 population storage")
     ap.add_argument("--tune-reps", type=int, default=10,
                     help="Poisson solves per candidate plan in the autotuner (0: default plan)")
-    ap.add_argument("--plan", default=None, metavar="FFT_ELEMS,TRI_ROWS,FAMILY",
-                    help="use this Poisson plan instead of autotuning (family 0 one-tile, 1 persistent, "
-                         "2 row-pair), e.g. to profile the kernels the autotuner picked")
+    ap.add_argument("--plan", default=None, metavar="FFT_ELEMS,TRI_ROWS,FAMILY[,LBM_WROWS]",
+                    help="use this Poisson plan (family 0 one-tile, 1 persistent, 2 row-pair) and, if given, "
+                         "this LBM tile shape (0, 2, 4 rows per row-kernel tile) instead of autotuning, e.g. "
+                         "to profile the kernels the autotuners picked")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB device copy three times (the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters)")
@@ -244,8 +245,9 @@ def main():
     print(f"bench: {args.config} ({desc}), {precision}, warmup {args.warmup}, steps {args.steps}",
           file=sys.stderr, flush=True)
     s = ekpnp.Solver(params, precision=precision, compute_fp64=(args.compute == "fp64"))
-    if args.plan:
-        fe, tr, fam = (int(v) for v in args.plan.split(","))
+    plan_vals = [int(v) for v in args.plan.split(",")] if args.plan else []
+    if plan_vals:
+        fe, tr, fam = plan_vals[:3]
         s.set_poisson_plan(fe, tr, fam)
         plan = s.poisson_plan()
     else:
@@ -254,6 +256,12 @@ def main():
     s.initialization(ekpnp.COLD_NOISE, seed)
     s.init_equilibrium()
     s.step(args.warmup)
+    # the row kernel's tile shape, timed on the warmed-up state (not advanced)
+    if len(plan_vals) > 3:
+        s.set_lbm_tiles(plan_vals[3])
+        lbm_tiles = {"wrows": s.lbm_tiles()}
+    else:
+        lbm_tiles = s.autotune_lbm(args.tune_reps) if args.tune_reps > 0 else {"wrows": s.lbm_tiles()}
     # The timed call replays cached hipGraphs: an untimed call of the same
     # length captures them first (two for odd counts, which end on the other
     # lattice parity).  FP32 storage re-centres its references on a schedule
@@ -319,7 +327,7 @@ def main():
         "data": "synthetic (seeded cold start: c0 = 1 with U(-1e-3, 1e-3) noise)",
         "config": {"workload": desc, "nodes": nodes, "precision": precision,
                    "collision_arithmetic": "f64" if (precision == "fp64" or args.compute == "fp64") else "f32",
-                   "poisson_plan": plan},
+                   "poisson_plan": plan, "lbm_tiles": lbm_tiles},
         "hbm_frac_coupled": round(mlups * 1e6 * coupled_b / (HBM_PEAK_GBS * 1e9), 4),
         "stages_ms": {k: round(v, 5) for k, v in st.items()},
         "roofline": {

@@ -89,6 +89,8 @@ struct ekpnp_ctx {
   int2* d_lbm_split = nullptr;   // lbm_split on the device
   std::vector<int2> lbm_split;   // LBM tiles for the step: row-kernel interior, row-kernel wall, others
   int n_lbm_row = 0, n_lbm_wall = 0;
+  int lbm_wrows = 0;             // interior tile shape of the row kernel (0: consecutive blocks)
+  size_t lbm_split_cap = 0;      // capacity of d_lbm_split (the largest shape)
   // the coupled step updates the row kernel's wall tiles inside the
   // tridiagonal launch (set with the Poisson plan)
   bool walls_in_tri = false;
@@ -274,6 +276,7 @@ StepBuffers make_buffers(ekpnp_ctx* c) {
   b.n_lbm_wall_tiles = c->n_lbm_wall;
   b.lbm_gen_tiles = c->d_lbm_split + c->n_lbm_row + c->n_lbm_wall;
   b.n_lbm_gen_tiles = (int)c->lbm_split.size() - c->n_lbm_row - c->n_lbm_wall;
+  b.lbm_wrows = c->lbm_wrows;
   return b;
 }
 
@@ -303,6 +306,64 @@ void release(ekpnp_ctx* c) {
 void drop_graphs(ekpnp_ctx* c) {
   for (auto& kv : c->graphs) (void)hipGraphExecDestroy(kv.second);
   c->graphs.clear();
+}
+
+// The step's split of the LBM tiles (FP32 storage, FP64 arithmetic):
+// channels with nx a multiple of 64 run on the row kernel, the others on
+// k_lbm.  The row kernel's wall launch takes the tiles of consecutive 64-node
+// blocks that hold a wall row; its interior launch those that hold an
+// interior row (wrows = 0) or tiles of wrows interior rows x 4 / wrows
+// blocks (wrows = 2, 4).  EKPNP_LBM_ROWS=0 in the environment keeps every
+// tile on k_lbm.
+void build_lbm_split(ekpnp_ctx* c, int wrows) {
+  const char* env = std::getenv("EKPNP_LBM_ROWS");
+  const bool use_rows = !(env && env[0] == '0');
+  std::vector<int2> wall, gen;
+  c->lbm_split.clear();
+  for (const int2& t : c->lbm_tiles) {
+    const ChanDesc& d = c->desc[t.x];
+    if (!use_rows || d.nx % 64 != 0) {
+      gen.push_back(t);
+      continue;
+    }
+    const int last = std::min(t.y + LBM_WG, d.nx * d.ny) - 1;  // last node of the tile
+    const int ya = t.y / d.nx, yb = last / d.nx;
+    if (ya == 0 || yb == d.ny - 1) wall.push_back(t);
+    if (wrows == 0 && !(ya == yb && (ya == 0 || ya == d.ny - 1))) c->lbm_split.push_back(t);
+  }
+  if (wrows > 0 && use_rows)
+    for (int i = 0; i < c->nchan; ++i) {
+      const ChanDesc& d = c->desc[i];
+      if (d.nx % 64 != 0) continue;
+      const int nblk = d.nx / 64, bpt = (LBM_WG / 64) / wrows;
+      for (int y0 = 1; y0 <= d.ny - 2; y0 += wrows)
+        for (int k0 = 0; k0 < nblk; k0 += bpt) c->lbm_split.push_back(make_int2(i, y0 * d.nx + 64 * k0));
+    }
+  c->n_lbm_row = (int)c->lbm_split.size();
+  c->n_lbm_wall = (int)wall.size();
+  c->lbm_split.insert(c->lbm_split.end(), wall.begin(), wall.end());
+  c->lbm_split.insert(c->lbm_split.end(), gen.begin(), gen.end());
+  c->lbm_wrows = wrows;
+}
+
+// a new interior tile shape for the row kernel: table, buffers, graphs
+hipError_t apply_lbm_tiles(ekpnp_ctx* c, int wrows) {
+  hipError_t e = hipStreamSynchronize(c->stream);  // no launch still reads the table
+  if (e != hipSuccess) return e;
+  build_lbm_split(c, wrows);
+  if (c->lbm_split.size() > c->lbm_split_cap) return hipErrorInvalidValue;
+  e = hipMemcpy(c->d_lbm_split, c->lbm_split.data(), c->lbm_split.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return e;
+  const StepBuffers nb = make_buffers(c);
+  c->sb.lbm_row_tiles = nb.lbm_row_tiles;
+  c->sb.n_lbm_row_tiles = nb.n_lbm_row_tiles;
+  c->sb.lbm_wall_tiles = nb.lbm_wall_tiles;
+  c->sb.n_lbm_wall_tiles = nb.n_lbm_wall_tiles;
+  c->sb.lbm_gen_tiles = nb.lbm_gen_tiles;
+  c->sb.n_lbm_gen_tiles = nb.n_lbm_gen_tiles;
+  c->sb.lbm_wrows = wrows;
+  drop_graphs(c);
+  return hipSuccess;
 }
 
 // Poisson work decomposition.  x-FFT: a work-group transforms whole rows,
@@ -815,30 +876,11 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     for (long long s = 0; s < nn; s += LBM_WG) c->lbm_tiles.push_back(make_int2(i, (int)s));
   }
   c->first_tile[n_channels] = (int)c->lbm_tiles.size();
-  // the step's split (FP32 storage, FP64 arithmetic): channels with nx a
-  // multiple of 64 run on the row kernel, its interior launch taking the
-  // tiles that hold an interior row and its wall launch those that hold a
-  // wall row; the others run on k_lbm.  EKPNP_LBM_ROWS=0 in the environment
-  // keeps every tile on k_lbm.
-  {
-    const char* env = std::getenv("EKPNP_LBM_ROWS");
-    const bool use_rows = !(env && env[0] == '0');
-    std::vector<int2> wall, gen;
-    for (const int2& t : c->lbm_tiles) {
-      const ChanDesc& d = c->desc[t.x];
-      if (!use_rows || d.nx % 64 != 0) {
-        gen.push_back(t);
-        continue;
-      }
-      const int last = std::min(t.y + LBM_WG, d.nx * d.ny) - 1;  // last node of the tile
-      const int ya = t.y / d.nx, yb = last / d.nx;
-      if (ya == 0 || yb == d.ny - 1) wall.push_back(t);
-      if (!(ya == yb && (ya == 0 || ya == d.ny - 1))) c->lbm_split.push_back(t);
-    }
-    c->n_lbm_row = (int)c->lbm_split.size();
-    c->n_lbm_wall = (int)wall.size();
-    c->lbm_split.insert(c->lbm_split.end(), wall.begin(), wall.end());
-    c->lbm_split.insert(c->lbm_split.end(), gen.begin(), gen.end());
+  // the step's split for every interior tile shape (the device table holds
+  // the largest), the default shape last
+  for (int w : {2, 4, 0}) {
+    build_lbm_split(c, w);
+    c->lbm_split_cap = std::max(c->lbm_split_cap, c->lbm_split.size());
   }
   // Poisson tile tables: capacity for the finest decomposition (one row per
   // FFT tile, two modes per tridiagonal work-group)
@@ -876,7 +918,7 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
     }
     if ((e = dalloc(c, &c->d_desc, n_channels)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_lbm_tiles, c->lbm_tiles.size())) != hipSuccess) return e;
-    if ((e = dalloc(c, &c->d_lbm_split, c->lbm_split.size())) != hipSuccess) return e;
+    if ((e = dalloc(c, &c->d_lbm_split, c->lbm_split_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_fft_tiles, c->fft_tiles_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_tri_tiles, c->tri_tiles_cap)) != hipSuccess) return e;
     if ((e = dalloc(c, &c->d_first_tile, c->first_tile.size())) != hipSuccess) return e;
@@ -1636,6 +1678,85 @@ int ekpnp_get_poisson_plan(const ekpnp_ctx* ctx, int* out) {
   out[2] = ctx->fft_persist;
   out[3] = (ctx->walls_in_tri && ctx->opt.precision == 0 && ctx->opt.compute_fp64 != 0) ? 1 : 0;
   return EKPNP_OK;
+}
+
+int ekpnp_set_lbm_tiles(ekpnp_ctx* ctx, int wrows) {
+  if (!ctx) return EKPNP_ERR_INVALID;
+  if (wrows != 0 && wrows != 2 && wrows != 4) return fail(ctx, EKPNP_ERR_INVALID, "wrows must be 0, 2 or 4");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, apply_lbm_tiles(ctx, wrows));
+  return EKPNP_OK;
+}
+
+int ekpnp_get_lbm_tiles(const ekpnp_ctx* ctx, int* wrows) {
+  if (!ctx || !wrows) return EKPNP_ERR_INVALID;
+  *wrows = ctx->lbm_wrows;
+  return EKPNP_OK;
+}
+
+// Times the row kernel's interior launch for each tile shape on the current
+// state without advancing it: a trial reads the current lattice and writes
+// the other one (the next step overwrites it) and the charge density, which
+// is saved and put back; no macroscopic planes.
+int ekpnp_autotune_lbm(ekpnp_ctx* ctx, int reps, double* out) {
+  const ApiRange range_("ekpnp_autotune_lbm");
+  if (!ctx || reps <= 0) return EKPNP_ERR_INVALID;
+  if (!ctx->has_pops) return fail(ctx, EKPNP_ERR_STATE, "autotune_lbm: no populations (call init_equilibrium first)");
+  HIP_OR_FAIL(ctx, hipSetDevice(ctx->opt.device));
+  HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+  const bool rows = ctx->opt.precision == EKPNP_POP_FP32 && ctx->opt.compute_fp64 != 0 && ctx->n_lbm_row > 0;
+  if (!rows) {  // no channel on the row kernel: nothing to choose
+    if (out) {
+      out[0] = ctx->lbm_wrows;
+      out[1] = 0.0;
+    }
+    return EKPNP_OK;
+  }
+  double* keep = nullptr;
+  HIP_OR_FAIL(ctx, hipMalloc((void**)&keep, ctx->total_nodes * sizeof(double)));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  auto finish = [&](int rc) {
+    for (auto e : ev)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipFree(keep);
+    return rc;
+  };
+  if (hipMemcpyAsync(keep, ctx->d_rho_e, ctx->total_nodes * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream) !=
+          hipSuccess ||
+      hipEventCreate(&ev[0]) != hipSuccess || hipEventCreate(&ev[1]) != hipSuccess)
+    return finish(fail(ctx, EKPNP_ERR_HIP, "autotune_lbm set-up failed"));
+  const int shapes[3] = {0, 2, 4};
+  double best_ms[3] = {1e30, 1e30, 1e30};
+  // three passes over the shapes, each shape's fastest pass counting
+  for (int pass = 0; pass < 3; ++pass)
+    for (int is = 0; is < 3; ++is) {
+      hipError_t e = apply_lbm_tiles(ctx, shapes[is]);
+      const void* src = ctx->lat[ctx->cur];
+      void* dst = ctx->lat[ctx->cur ^ 1];
+      for (int w = 0; w < 2 && e == hipSuccess; ++w) e = launch_lbm_rows_interior(ctx->sb, src, dst, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ev[0], ctx->stream);
+      for (int k = 0; k < reps && e == hipSuccess; ++k) e = launch_lbm_rows_interior(ctx->sb, src, dst, ctx->stream);
+      if (e == hipSuccess) e = hipEventRecord(ev[1], ctx->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(ev[1]);
+      float ms = 0.0f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ev[0], ev[1]);
+      if (e != hipSuccess)
+        return finish(fail(ctx, EKPNP_ERR_HIP, std::string("autotune_lbm: ") + hipGetErrorString(e)));
+      best_ms[is] = std::min(best_ms[is], (double)ms / reps);
+    }
+  int best = 0;
+  for (int is = 1; is < 3; ++is)
+    if (best_ms[is] < best_ms[best]) best = is;
+  hipError_t e = apply_lbm_tiles(ctx, shapes[best]);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->d_rho_e, keep, ctx->total_nodes * sizeof(double), hipMemcpyDeviceToDevice, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return finish(fail(ctx, EKPNP_ERR_HIP, std::string("autotune_lbm: ") + hipGetErrorString(e)));
+  if (out) {
+    out[0] = shapes[best];
+    out[1] = best_ms[best];
+  }
+  return finish(EKPNP_OK);
 }
 
 int ekpnp_get_params(ekpnp_ctx* ctx, int channel, ekpnp_channel_params* out) {

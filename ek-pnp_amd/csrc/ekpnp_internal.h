@@ -95,6 +95,7 @@ struct StepBuffers {
   int n_lbm_wall_tiles;
   const int2* lbm_gen_tiles;
   int n_lbm_gen_tiles;
+  int lbm_wrows;                // row-kernel interior tiles: 0 consecutive blocks, else rows per tile (2, 4)
   const int4* fft_tiles;        // (channel, first row, rows, 0) per FFT work-group
   int n_fft_tiles;
   int n_fft_pow2;               // the first n_fft_pow2 tiles are power-of-two rows
@@ -134,6 +135,7 @@ struct StepBuffers {
 // for FP32 storage
 // walls = false: the row kernel's wall tiles are left out (for
 // launch_poisson_tri_walls to update them)
+hipError_t launch_lbm_rows_interior(const StepBuffers& b, const void* src, void* dst, hipStream_t s);
 hipError_t launch_lbm(int precision, bool compute64, bool from_q, bool macro, const StepBuffers& b,
                       const void* src, void* dst, hipStream_t s, bool walls = true);
 hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s);

@@ -798,17 +798,27 @@ __device__ __forceinline__ void lbm_row_node(const StepBuffers& b, const ChanDes
 // launch updates the waves on wall rows, the other one the waves on interior
 // rows (one kernel holding both paths would need the registers of both); a
 // tile holding both kinds of row is in both launches.
+// Tile shapes: wrows = 0, the work-group's waves take consecutive 64-node
+// blocks from the tile's first node on (across row ends); wrows = 2 or 4,
+// the tile is wrows rows x (4 / wrows) blocks whose first node is the tile's
+// (waves past a row's end or past the last row return).
 template <bool WALL>
 __device__ __forceinline__ void lbm_rows_tile(const StepBuffers& b, int2 tile, const float* __restrict__ src,
-                                              float* __restrict__ dst, bool macro) {
+                                              float* __restrict__ dst, bool macro, int wrows) {
   const ChanDesc* __restrict__ cd = b.chans + tile.x;
   if (!cd->active) return;  // converged channel, frozen by the residual driver
   const int nx = cd->nx, ny = cd->ny;
-  const int node0 = tile.y + ((int)threadIdx.x & ~63);  // the wave's first node
-  if (node0 >= nx * ny) return;                        // whole waves past the channel's last row
+  const int w = (int)threadIdx.x >> 6;
+  int node0 = tile.y + 64 * w;  // the wave's first node
+  if (wrows > 0) {
+    const int bpt = (LBM_WG / 64) / wrows, r = w / bpt, bo = w - r * bpt;
+    if (tile.y % nx + 64 * bo >= nx) return;  // a block past the row's end
+    node0 = tile.y + r * nx + 64 * bo;
+  }
+  if (node0 >= nx * ny) return;  // whole waves past the channel's last row
   const int y = __builtin_amdgcn_readfirstlane(node0 / nx);
   if ((y == 0 || y == ny - 1) != WALL) return;         // the other launch's wave
-  const int x = tile.y + (int)threadIdx.x - y * nx;
+  const int x = node0 + ((int)threadIdx.x & 63) - y * nx;
   const int k = __builtin_amdgcn_readfirstlane((node0 - y * nx) >> 6);  // the wave's reference block
   const float* __restrict__ P = (cd->swap ? dst : src) + cd->pop_off;
   float* __restrict__ Q = (cd->swap ? const_cast<float*>(src) : dst) + cd->pop_off;
@@ -820,7 +830,7 @@ template <bool WALL>
 __global__ __launch_bounds__(LBM_WG) void k_lbm_rows(StepBuffers b, const int2* __restrict__ tiles,
                                                     const float* __restrict__ src, float* __restrict__ dst,
                                                     int macro) {
-  lbm_rows_tile<WALL>(b, tiles[xcd_remap(blockIdx.x, gridDim.x)], src, dst, macro != 0);
+  lbm_rows_tile<WALL>(b, tiles[xcd_remap(blockIdx.x, gridDim.x)], src, dst, macro != 0, WALL ? 0 : b.lbm_wrows);
 }
 
 
@@ -2187,7 +2197,7 @@ __global__ __launch_bounds__(TRI_WG) void k_tridiag_walls(StepBuffers b, const f
   if (i >= nw)
     tridiag_tile<LMAX>(b, b.tri_tiles[xcd_remap(i - nw, nt)], sA, sB);
   else if (i < b.n_lbm_wall_tiles)
-    lbm_rows_tile<true>(b, b.lbm_wall_tiles[i], src, dst, MACRO);
+    lbm_rows_tile<true>(b, b.lbm_wall_tiles[i], src, dst, MACRO, 0);
 }
 
 // table[j][k] = cprime_closed(j L, k kdy_scale) for chunks j = 1..C-1 (row
@@ -2546,6 +2556,15 @@ static hipError_t lbm_dispatch(bool from_q, bool macro, bool walls, const StepBu
       hipLaunchKernelGGL((k_lbm_rows<false>), dim3(b.n_lbm_row_tiles), block, 0, s, b, b.lbm_row_tiles, fs, fd,
                          macro ? 1 : 0);
   }
+  return hipGetLastError();
+}
+
+// the row kernel's interior launch alone (no macroscopic planes): the LBM
+// tile-shape autotuner's trial
+hipError_t launch_lbm_rows_interior(const StepBuffers& b, const void* src, void* dst, hipStream_t s) {
+  if (b.n_lbm_row_tiles > 0)
+    hipLaunchKernelGGL((k_lbm_rows<false>), dim3(b.n_lbm_row_tiles), dim3(LBM_WG), 0, s, b, b.lbm_row_tiles,
+                       static_cast<const float*>(src), static_cast<float*>(dst), 0);
   return hipGetLastError();
 }
 

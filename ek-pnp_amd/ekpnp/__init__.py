@@ -125,6 +125,9 @@ SIGNATURES = [
     ("ekpnp_set_poisson_plan", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     ("ekpnp_autotune", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     ("ekpnp_get_poisson_plan", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+    ("ekpnp_set_lbm_tiles", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("ekpnp_get_lbm_tiles", ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_int)]),
+    ("ekpnp_autotune_lbm", ctypes.c_int, [_VP, ctypes.c_int, _DP]),
     ("ekpnp_num_channels", ctypes.c_int, [_VP]),
     ("ekpnp_steps_done", ctypes.c_longlong, [_VP]),
     ("ekpnp_device_bytes", ctypes.c_size_t, [_VP]),
@@ -460,6 +463,24 @@ class Solver:
         self._check(lib().ekpnp_autotune(self._ctx, int(reps), _dptr(out)))
         return {"fft_elems": int(out[0]), "tri_rows": int(out[1]), "poisson_ms": float(out[2]),
                 "persistent": int(out[3]) == 1, "fft_kernels": FFT_KERNELS[int(out[3])]}
+
+    def set_lbm_tiles(self, wrows: int) -> None:
+        """Tile shape of the row collide-stream kernel: 0 four consecutive
+        64-node blocks per work-group (default), 2 or 4 that many interior rows
+        x 4 / wrows blocks.  Results are identical for every shape."""
+        self._check(lib().ekpnp_set_lbm_tiles(self._ctx, int(wrows)))
+
+    def lbm_tiles(self) -> int:
+        out = ctypes.c_int(0)
+        self._check(lib().ekpnp_get_lbm_tiles(self._ctx, ctypes.byref(out)))
+        return int(out.value)
+
+    def autotune_lbm(self, reps: int = 10) -> dict:
+        """Times the row kernel's interior launch per tile shape on the current
+        state (not advanced) and keeps the fastest (needs populations)."""
+        out = np.zeros(2, dtype=np.float64)
+        self._check(lib().ekpnp_autotune_lbm(self._ctx, int(reps), _dptr(out)))
+        return {"wrows": int(out[0]), "ms": float(out[1])}
 
     def poisson_plan(self) -> dict:
         """The plan in force (ekpnp_get_poisson_plan)."""

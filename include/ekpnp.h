@@ -359,6 +359,23 @@ int ekpnp_set_poisson_plan(ekpnp_ctx* ctx, int fft_elems, int tri_rows, int fft_
  * per solve, fft_persistent}.  No counterpart in the reference (cuFFT plans are
  * fixed by main.cu:113). */
 int ekpnp_autotune(ekpnp_ctx* ctx, int reps, double* out);
+/* Tile shape of the row collide-stream kernel (FP32 storage with FP64
+ * arithmetic, channels with nx a multiple of 64; a 256-lane work-group is
+ * four waves of 64 nodes of one row each): wrows = 0 (default) gives a
+ * work-group four consecutive 64-node blocks in node order, wrows = 2 or 4
+ * gives it wrows interior rows x 4 / wrows blocks.  Results do not depend on
+ * the shape, bit for bit.  Replaces the reference's fixed tiles of one row
+ * x nThreads nodes (LBM.cu:311-314). */
+int ekpnp_set_lbm_tiles(ekpnp_ctx* ctx, int wrows);
+int ekpnp_get_lbm_tiles(const ekpnp_ctx* ctx, int* wrows);
+/* Times the row kernel's interior launch for each tile shape (reps launches
+ * each, three passes, HIP events on the context's stream) on the current
+ * state without advancing it (it writes only the other lattice and the
+ * charge density, which is put back), and keeps the fastest; needs
+ * populations.  out (may be NULL, else 2 doubles) receives {wrows, ms per
+ * launch}; a context with no channel on the row kernel keeps its shape (ms
+ * 0).  No counterpart in the reference. */
+int ekpnp_autotune_lbm(ekpnp_ctx* ctx, int reps, double* out);
 /* The plan in force: out[0] fft_elems, out[1] tri_rows, out[2] persistent FFT
  * kernels in use, out[3] whether a coupled step updates the collide-stream's
  * wall rows (the rows y = 0 and ny - 1 of channels with nx a multiple of 64,
