@@ -6,7 +6,7 @@ phi_k) (k = 1..4, seed 4), kappa h 64, E_x 1e-4, tau 0.8, FP32 populations,
 is run three ways for the same number of steps: cold (uniform c0 + noise),
 surrogate-warm and physics-baseline-warm.  A warm start broadcasts the
 profiles predicted for the channel's mean parameters (zeta0, kappa h, E_x,
-n_y) along x; it is then judged exactly as in the batched experiment: 100
+n_y) along x; it is then accepted or rejected as in the batched experiment: 100
 settling steps, then the u_x residual over a 200-step probe must be at most
 `accept`, else the channel restarts from its cold fields (the fallback).
 
