@@ -101,3 +101,36 @@ def test_committed_network_meets_the_fivefold_target():
     assert not rejected
     assert (warm < cold).all()
     assert np.median(gain) >= 5.0
+
+
+@pytest.mark.skipif(not os.path.exists(NET_PATH), reason="no committed warm-start network")
+@pytest.mark.timeout(900)
+def test_batched_sweep_runs_to_the_residual_from_warm_starts():
+    # the batched sweep (256 channels of seed 2: n_x 32..256, n_y 64..1024,
+    # 18.2 M nodes) in one context, warm-started from the committed network
+    # and finished by the plain method to the 1e-8 u_x residual: every
+    # prediction kept, every channel converged (measured makespan 158 500
+    # steps; cold starts of its n_y >= 896 channels take millions), fields
+    # finite and the fluid mass kept
+    net = S.load_net(NET_PATH)
+    ps, _ = S.sample_channels(256, 2)
+    print("case batched sweep to the residual, warm-started", flush=True)
+    predict = T.predictor(net, True)
+    # the warm start's density: the osmotic profile of the predicted columns
+    prof = predict(ps)
+    mass0 = np.array([p.nx * S.osmotic_rho(p, *prof[c][:3]).sum() for c, p in enumerate(ps)])
+    s, warm, wall, res, rejected = T.run_warm(ps, predict, "fp32", 2, 1e-8, 100, 400_000, 200, ACCEPT,
+                                              chunk=20_000)
+    red = s.reductions()
+    mass = red[:, ekpnp.RED_NAMES.index("mass")]
+    s.close()
+    net.close()
+    print("converged", int((warm > 0).sum()), "median", float(np.median(warm[warm > 0])), "makespan",
+          int(warm.max()), "wall", round(wall, 1), "rejected", rejected, flush=True)
+    assert not rejected
+    assert (warm > 0).all()
+    assert (red[:, ekpnp.RED_NAMES.index("nonfinite")] == 0).all()
+    # every channel keeps the fluid mass it was started with
+    drift = np.abs(mass / mass0 - 1.0)
+    print("largest mass drift", float(drift.max()), flush=True)
+    assert drift.max() < 1e-6
