@@ -986,20 +986,17 @@ int ekpnp_create(const ekpnp_channel_params* channels, int n_channels, const ekp
   int n_cu = 256;
   if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, o.device) == hipSuccess && n_cu > 0)
     c->n_cu = n_cu;
-  // default plan (ekpnp_autotune refines it): rows of nx = 4096 run fastest
-  // in the row-pair kernels (two rows per work-group; when the channels'
-  // widths differ, two rows per work-group in the one-tile kernels), wider
-  // rows one per work-group in the one-tile kernels, narrower rows in the
-  // persistent pipelined kernels with up to 1024 values per tile
+  // default plan (ekpnp_autotune refines it): rows of nx = 4096 and 8192
+  // run fastest in the row-pair kernels (two rows per work-group; when the
+  // channels' widths differ, the one-tile kernels with as many rows as fit
+  // 4096 values), narrower rows in the persistent pipelined kernels with up
+  // to 1024 values per tile
   int max_n = 0;
   for (const auto& d : c->desc)
     if (d.pow2) max_n = std::max(max_n, d.nx / 2);
-  if (max_n == 2048) {
+  if (max_n >= 2048) {
     c->fft_elems = 4096;
     c->fft_persist = 2;
-  } else if (max_n > 2048) {
-    c->fft_elems = 2048;
-    c->fft_persist = 0;
   } else {
     c->fft_elems = 1024;
     c->fft_persist = 1;

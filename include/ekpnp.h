@@ -339,8 +339,10 @@ int ekpnp_load_checkpoint(ekpnp_ctx* ctx, const char* path);
 
 /* ---- work decomposition ---------------------------------------------------- */
 /* Poisson plan: x-FFT work-groups hold up to fft_elems complex values (whole
- * rows; default 2048), tridiagonal columns are cut into chunks of about
- * tri_rows rows (default 16); fft_persistent selects the x-FFT kernels:
+ * rows), tridiagonal columns are cut into chunks of about tri_rows rows
+ * (default 16); ekpnp_create picks fft_elems = 4096 with the row-pair kernels
+ * when the widest power-of-two rows have nx = 4096 or 8192, else 1024 with
+ * the persistent kernels; fft_persistent selects the x-FFT kernels:
  * 0 one tile per work-group, 1 the persistent software-pipelined kernels
  * where the tiles allow them (fft_elems <= 2048, nx <= 4096), 2 the row-pair
  * kernels (two rows per work-group transformed one after the other, the
