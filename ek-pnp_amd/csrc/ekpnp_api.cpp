@@ -508,12 +508,14 @@ hipError_t build_poisson_plan(ekpnp_ctx* c, int fft_elems, int tri_rows, int fft
     }
   }
   // Wall tiles on the tridiagonal launch when they fit the slots its tiles
-  // leave free (256-lane work-groups, three per CU at its register count):
-  // the launch then keeps its number of rounds.  EKPNP_WALLS_IN_TRI=0 in the
-  // environment keeps them on a launch of their own.
+  // leave free (256-lane work-groups, as many per CU as its registers allow,
+  // three on gfx950): the launch then keeps its number of rounds.
+  // EKPNP_WALLS_IN_TRI=0 in the environment keeps them on a launch of their
+  // own.
   {
     const char* env = std::getenv("EKPNP_WALLS_IN_TRI");
-    const long long slots = 3LL * c->n_cu;
+    const int per_cu = tridiag_walls_blocks_per_cu();
+    const long long slots = (long long)(per_cu > 0 ? per_cu : 3) * c->n_cu;
     const long long nt = (long long)c->tri_tiles.size(), nw = (c->n_lbm_wall + 7) & ~7;
     c->walls_in_tri = !(env && env[0] == '0') && nw > 0 && (nt + nw + slots - 1) / slots == (nt + slots - 1) / slots;
   }

@@ -147,6 +147,7 @@ hipError_t launch_poisson_inv(const StepBuffers& b, hipStream_t s);
 // resident one-tile x-FFT work-groups per CU with twiddles in LDS (twl) or
 // not, lds bytes of dynamic LDS each (registers and LDS both counted); 0 on error
 int fft_one_tile_blocks_per_cu(bool twl, unsigned lds);
+int tridiag_walls_blocks_per_cu();
 // chunk-entry Thomas coefficients of channel chan for its current tri_C, tri_L
 // chunk-entry Thomas coefficients of every channel (max_entries: the
 // largest tri_C x nk over the channels)

@@ -2602,6 +2602,14 @@ hipError_t launch_poisson_fwd(const StepBuffers& b, hipStream_t s) {
   return hipGetLastError();
 }
 
+// resident work-groups per CU of the coupled step's tridiagonal launch
+int tridiag_walls_blocks_per_cu() {
+  int n = 0;
+  const hipError_t e =
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tridiag_walls<TRI_LMAX, false>, TRI_WG, 0);
+  return e == hipSuccess ? n : 0;
+}
+
 int fft_one_tile_blocks_per_cu(bool twl, unsigned lds) {
   int n = 0;
   hipError_t e = twl ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_fft_fwd<true, true>, FFT_WG_ROWS, lds)
