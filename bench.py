@@ -90,15 +90,20 @@ def calibrate_copy():
                       "ms_per_copy": round(r["ms_per_copy"], 4), "GBps": round(r["GBps"], 1)}), flush=True)
 
 
-def measured_traffic(config, precision):
-    """HBM bytes per LBM stage (its interior- and wall-row launches) from the committed counter summary of this
-    workload (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or
-    None when no summary exists for it."""
+def measured_traffic(config, precision, wrows=0):
+    """HBM bytes per launch of the row kernel's interior rows, for the tile
+    shape the run used, from the committed counter summary of this workload
+    (FETCH_SIZE and WRITE_SIZE scaled by the copy calibration), or None when
+    no summary exists for it."""
     path = os.path.join(ROOT, "profiles", f"k_lbm_{config}_{precision}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        d = json.load(f)
+    by_shape = d.get("bytes_per_launch_by_tile_rows", {})
+    if str(wrows) in by_shape:
+        return by_shape[str(wrows)]
+    return d.get("bytes_per_launch") if wrows == 0 else None
 
 
 def timed_start(warmup, steps, precision):
@@ -337,7 +342,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.config, precision),
+            "traffic": measured_traffic(args.config, precision, lbm_tiles["wrows"]),
         },
         "finite": bool(red[:, 6].sum() == 0),
         "copy_ceiling_gbps": round(copy["GBps"], 1),
