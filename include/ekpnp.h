@@ -373,10 +373,11 @@ int ekpnp_get_lbm_tiles(const ekpnp_ctx* ctx, int* wrows);
 /* Times the row kernel's interior launch for each tile shape (reps launches
  * each, three passes, HIP events on the context's stream) on the current
  * state without advancing it (it writes only the other lattice and the
- * charge density, which is put back), and keeps the fastest; needs
- * populations.  out (may be NULL, else 2 doubles) receives {wrows, ms per
- * launch}; a context with no channel on the row kernel keeps its shape (ms
- * 0).  No counterpart in the reference. */
+ * charge density, which is put back), and keeps the fastest (another shape
+ * replaces the default only when at least 1 % faster); needs populations.
+ * out (may be NULL, else 2 doubles) receives {wrows, ms per launch}; a
+ * context with no channel on the row kernel keeps its shape (ms 0).  No
+ * counterpart in the reference. */
 int ekpnp_autotune_lbm(ekpnp_ctx* ctx, int reps, double* out);
 /* The plan in force: out[0] fft_elems, out[1] tri_rows, out[2] persistent FFT
  * kernels in use, out[3] whether a coupled step updates the collide-stream's
