@@ -1846,13 +1846,14 @@ __global__ __launch_bounds__(FFT_WG_ROWS) void k_fft_inv2(StepBuffers b, const i
 // twiddle table's quadrant-table values, row A, row B, so that filling the
 // table waits for its own loads only and row A's first use for row A's.  The
 // wall lifts of rows 1 and ny-2 are read where they are applied, by the two
-// tiles holding those rows.  Same arithmetic as the one-tile kernels with the
-// two-level table.  Four waves per SIMD (two work-groups per CU, the LDS
-// limit) cap the registers at 128.
+// tiles holding those rows.  The operations of the one-tile kernels with the
+// two-level table (results equal to FP64 rounding: the compiler contracts
+// multiply-adds per kernel).  Four waves per SIMD (two work-groups per CU,
+// the LDS limit) cap the registers at 128.
 // ---------------------------------------------------------------------------
 template <int S>
-__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_fwd2w(StepBuffers b, const int4* __restrict__ tiles,
-                                                              const ChanDesc* __restrict__ chans) {
+__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_fwd2w(
+    StepBuffers b, const int4* __restrict__ tiles, const ChanDesc* __restrict__ chans) {
   extern __shared__ double2 lds[];
   constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, nx = 2 * N, quarter = N / 2;
   constexpr int NT2 = tw2_count(nx);
@@ -1928,8 +1929,8 @@ __global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))
 }
 
 template <int S>
-__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_inv2w(StepBuffers b, const int4* __restrict__ tiles,
-                                                              const ChanDesc* __restrict__ chans) {
+__global__ __launch_bounds__(FFT_WG_ROWS) __attribute__((amdgpu_waves_per_eu(4))) void k_fft_inv2w(
+    StepBuffers b, const int4* __restrict__ tiles, const ChanDesc* __restrict__ chans) {
   extern __shared__ double2 lds[];
   constexpr int WG = FFT_WG_ROWS, N = S * WG, lgN = clog2(N), lgnx = lgN + 1, nx = 2 * N, quarter = N / 2;
   constexpr int half = N / 2, P = half / WG + 1, NT2 = tw2_count(nx);  // pair slots per lane
